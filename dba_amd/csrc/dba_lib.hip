@@ -1,0 +1,776 @@
+// dba_lib.hip - C ABI of the MI355X DBA time step (libdba_hip.so).
+//
+// Host orchestration of the gfx950 kernels: instances own their device state,
+// groups merge instances of one model into one worklist, every step is
+//   step kernel (fused stencil + detector + dilation atomics)
+//   -> finalize kernel (parity flip, history, conserved shift / CFL dt,
+//      compaction of the next worklist),
+// on one HIP stream, optionally replayed from a captured hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/dba/dba.h"
+#include "common.h"
+#include "k_finalize.h"
+#include "k_ch.h"
+#include "k_k93.h"
+
+using namespace dba;
+
+namespace {
+
+thread_local std::string g_error;
+
+dba_status fail(dba_status code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(DBA_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));   \
+    } while (0)
+
+#define LAUNCH_CHECK()                                                                 \
+    do {                                                                               \
+        hipError_t e_ = hipGetLastError();                                             \
+        if (e_ != hipSuccess)                                                          \
+            return fail(DBA_ERR_LAUNCH, "kernel launch failed: %s", hipGetErrorString(e_)); \
+    } while (0)
+
+int nfields_of(int model) {
+    switch (model) {
+        case DBA_MODEL_CH: return 1;
+        case DBA_MODEL_DENDRITE_K93: return 2;
+        default: return 0;
+    }
+}
+
+int nsets_of(int model) {
+    return (model == DBA_MODEL_DENDRITE_K93 || model == DBA_MODEL_DENDRITE_K94) ? 2 : 1;
+}
+
+bool is_pow2_tile(int t) { return t == 8 || t == 16 || t == 32; }
+
+struct Instance {
+    int id = 0;
+    dba_params p{};
+    Inst h{};                    // host copy of the device descriptor
+    std::vector<void*> allocs;   // device allocations owned by this instance
+    int64_t footprint = 0;
+    int64_t ncell = 0;
+    Inst* d_self = nullptr;      // single-instance descriptor (downloads, checks)
+};
+
+struct Group {
+    std::vector<int> ids;
+    int model = 0, tile = 0, threads = 0;
+    Inst* d_insts = nullptr;
+    int32_t* d_counts = nullptr;
+    int32_t* d_offsets = nullptr;
+    uint32_t* d_done = nullptr;
+    int64_t total_tiles = 0;
+    int grid = 0;
+};
+
+struct GraphKey {
+    std::vector<int> ids;
+    int chunk;
+    bool operator<(const GraphKey& o) const { return chunk != o.chunk ? chunk < o.chunk : ids < o.ids; }
+};
+
+}  // namespace
+
+struct dba_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int next_id = 1;
+    std::map<int, Instance*> inst;
+    std::map<std::vector<int>, Group*> groups;
+    std::map<GraphKey, hipGraphExec_t> graphs;
+    double* staging = nullptr;
+    size_t staging_n = 0;
+    int* d_flag = nullptr;
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_step, ev_fin;
+    std::vector<hipEvent_t> ev_pool;
+    hipEvent_t markers[8] = {};
+};
+
+namespace {
+
+// Derived constants: the same C expressions the oracle evaluates.
+void derive_constants(const dba_params& p, Inst& h) {
+    h.invdx2 = 1.0 / (p.dx * p.dx);
+    h.inv6dx2 = 1.0 / (6.0 * p.dx * p.dx);
+    h.inv2dx = 1.0 / (2.0 * p.dx);
+    h.dt_tau = p.tau != 0.0 ? p.dt / p.tau : 0.0;
+    h.alpha_pi = p.alpha / M_PI;
+    double jf = (double)p.aniso_j;
+    h.cos_jt0 = std::cos(jf * p.theta0);
+    h.sin_jt0 = std::sin(jf * p.theta0);
+    h.k_epsp = -(p.eps_bar * jf * p.delta);
+}
+
+dba_status validate(const dba_params& p) {
+    if (nfields_of(p.model) == 0) return fail(DBA_ERR_INVALID, "model %d is not supported", p.model);
+    if (p.mode < 0 || p.mode > 2) return fail(DBA_ERR_INVALID, "mode %d invalid", p.mode);
+    if (p.nx <= 0 || p.ny <= 0 || p.nz <= 0) return fail(DBA_ERR_INVALID, "grid %dx%dx%d invalid", p.nx, p.ny, p.nz);
+    if (p.model == DBA_MODEL_CH || p.model == DBA_MODEL_DENDRITE_K93) {
+        if (p.nz != 1) return fail(DBA_ERR_INVALID, "model %d is 2D (nz must be 1)", p.model);
+        if (p.tile_x != p.tile_y || !is_pow2_tile(p.tile_x))
+            return fail(DBA_ERR_INVALID, "tile must be square 8, 16 or 32 (got %dx%d)", p.tile_x, p.tile_y);
+        if (p.nx % p.tile_x || p.ny % p.tile_y)
+            return fail(DBA_ERR_INVALID, "grid %dx%d must be a multiple of the tile %d", p.nx, p.ny, p.tile_x);
+        if (p.nx < p.tile_x * 1 || p.ny < p.tile_y) return fail(DBA_ERR_INVALID, "grid smaller than a tile");
+    }
+    if (p.model == DBA_MODEL_CH && p.stencil != 5 && p.stencil != 9)
+        return fail(DBA_ERR_INVALID, "CH stencil must be 5 or 9");
+    if (p.model == DBA_MODEL_CH && p.mode == DBA_MODE_CONSERVED && p.bc_x != DBA_BC_PERIODIC)
+        return fail(DBA_ERR_INVALID, "conserved DBA needs a periodic grid");
+    if (p.model == DBA_MODEL_DENDRITE_K93 && (p.aniso_j < 1 || p.aniso_j > 16))
+        return fail(DBA_ERR_INVALID, "aniso_j must be in [1,16]");
+    if (p.model == DBA_MODEL_DENDRITE_K93 && p.mode == DBA_MODE_CONSERVED)
+        return fail(DBA_ERR_INVALID, "conserved DBA is defined for Model B and KH only");
+    if (p.threads != 0 && p.threads != 64 && p.threads != 128 && p.threads != 256)
+        return fail(DBA_ERR_INVALID, "threads must be 0, 64, 128 or 256");
+    if (p.dx <= 0.0 || p.dt <= 0.0) return fail(DBA_ERR_INVALID, "dx and dt must be positive");
+    return DBA_OK;
+}
+
+struct Layout {
+    int ntx, nty, ntz, nt, nwords;
+    int64_t n;
+    int nf, nsets, naux;
+};
+
+Layout layout_of(const dba_params& p) {
+    Layout L{};
+    int tz = p.tile_z > 0 ? p.tile_z : 1;
+    L.ntx = (p.nx + p.tile_x - 1) / p.tile_x;
+    L.nty = (p.ny + p.tile_y - 1) / p.tile_y;
+    L.ntz = (p.nz + tz - 1) / tz;
+    L.nt = L.ntx * L.nty * L.ntz;
+    L.nwords = (L.nt + 63) / 64;
+    L.n = (int64_t)p.nx * p.ny * p.nz;
+    L.nf = nfields_of(p.model);
+    L.nsets = nsets_of(p.model);
+    L.naux = (p.model == DBA_MODEL_CH && p.mode != DBA_MODE_REGULAR) ? 1 : 0;
+    return L;
+}
+
+int64_t footprint_of(const dba_params& p) {
+    Layout L = layout_of(p);
+    int64_t b = 0;
+    b += 2 * (int64_t)L.nf * L.n * 8;        // double-buffered fields
+    b += (int64_t)L.naux * L.n * 8;          // stored neighbour state
+    b += 6 * (int64_t)L.nwords * 8;          // bitmaps
+    b += (int64_t)kMaxFields * L.nt * 8;     // tpart
+    b += 2 * (int64_t)L.nt * 8;              // tspeed
+    b += (int64_t)kMaxFields * 2 * L.nt * 8; // s_tile
+    b += 16 * 8 + 4 * 8;                     // scal + ctr
+    b += (int64_t)2 * std::max(0, p.history_cap) * 4;
+    b += (int64_t)L.nt * 4;                  // seg
+    b += sizeof(Inst);
+    return b;
+}
+
+template <typename T>
+dba_status dalloc(Instance* in, T** out, size_t count) {
+    void* ptr = nullptr;
+    size_t bytes = std::max<size_t>(count * sizeof(T), 8);
+    hipError_t e = hipMalloc(&ptr, bytes);
+    if (e != hipSuccess) return fail(DBA_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    e = hipMemset(ptr, 0, bytes);
+    if (e != hipSuccess) return fail(DBA_ERR_HIP, "hipMemset failed: %s", hipGetErrorString(e));
+    in->allocs.push_back(ptr);
+    *out = (T*)ptr;
+    return DBA_OK;
+}
+
+void free_instance(Instance* in) {
+    for (void* p : in->allocs) (void)hipFree(p);
+    in->allocs.clear();
+}
+
+void invalidate_groups(dba_ctx* ctx, int id) {
+    for (auto it = ctx->groups.begin(); it != ctx->groups.end();) {
+        if (std::find(it->first.begin(), it->first.end(), id) != it->first.end()) {
+            Group* g = it->second;
+            (void)hipFree(g->d_insts);
+            (void)hipFree(g->d_counts);
+            (void)hipFree(g->d_offsets);
+            (void)hipFree(g->d_done);
+            delete g;
+            it = ctx->groups.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    for (auto it = ctx->graphs.begin(); it != ctx->graphs.end();) {
+        if (std::find(it->first.ids.begin(), it->first.ids.end(), id) != it->first.ids.end()) {
+            (void)hipGraphExecDestroy(it->second);
+            it = ctx->graphs.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+
+int default_threads(int model, int tile) {
+    (void)model;
+    if (tile <= 8) return 64;
+    return 256;
+}
+
+dba_status get_group(dba_ctx* ctx, const std::vector<int>& ids, Group** out) {
+    auto it = ctx->groups.find(ids);
+    if (it != ctx->groups.end()) {
+        *out = it->second;
+        return DBA_OK;
+    }
+    Group* g = new Group();
+    g->ids = ids;
+    std::vector<Inst> hs;
+    for (int id : ids) {
+        Instance* in = ctx->inst[id];
+        hs.push_back(in->h);
+        g->total_tiles += in->h.nt;
+    }
+    Instance* first = ctx->inst[ids[0]];
+    g->model = first->p.model;
+    g->tile = first->p.tile_x;
+    g->threads = first->p.threads ? first->p.threads : default_threads(g->model, g->tile);
+    int64_t cap = 4096;
+    g->grid = (int)std::max<int64_t>(1, std::min<int64_t>(g->total_tiles, cap));
+    size_t n = ids.size();
+    HIP_TRY(hipMalloc(&g->d_insts, sizeof(Inst) * n));
+    HIP_TRY(hipMemcpy(g->d_insts, hs.data(), sizeof(Inst) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&g->d_counts, sizeof(int32_t) * n));
+    HIP_TRY(hipMalloc(&g->d_offsets, sizeof(int32_t) * (n + 1)));
+    HIP_TRY(hipMalloc(&g->d_done, sizeof(uint32_t)));
+    HIP_TRY(hipMemset(g->d_done, 0, sizeof(uint32_t)));
+    HIP_TRY(hipMemset(g->d_offsets, 0, sizeof(int32_t) * (n + 1)));
+    HIP_TRY(hipDeviceSynchronize());
+    ctx->groups[ids] = g;
+    *out = g;
+    return DBA_OK;
+}
+
+// ---------------------------------------------------------------- launches
+
+template <int TB, int NT>
+void launch_step_tb(const Group* g, hipStream_t s) {
+    dim3 grid(g->grid), block(NT);
+    int n = (int)g->ids.size();
+    switch (g->model) {
+        case DBA_MODEL_CH:
+            hipLaunchKernelGGL((ch_step_kernel<TB, NT>), grid, block, 0, s, g->d_insts, g->d_offsets, n);
+            break;
+        case DBA_MODEL_DENDRITE_K93:
+            hipLaunchKernelGGL((k93_step_kernel<TB, NT>), grid, block, 0, s, g->d_insts, g->d_offsets, n);
+            break;
+        default:
+            break;
+    }
+}
+
+template <int TB>
+void launch_step_nt(const Group* g, hipStream_t s) {
+    switch (g->threads) {
+        case 64: launch_step_tb<TB, 64>(g, s); break;
+        case 128: launch_step_tb<TB, 128>(g, s); break;
+        default: launch_step_tb<TB, 256>(g, s); break;
+    }
+}
+
+void launch_step(const Group* g, hipStream_t s) {
+    switch (g->tile) {
+        case 8: launch_step_nt<8>(g, s); break;
+        case 16: launch_step_nt<16>(g, s); break;
+        default: launch_step_nt<32>(g, s); break;
+    }
+}
+
+void launch_finalize(const Group* g, hipStream_t s, int rotate) {
+    hipLaunchKernelGGL((finalize_kernel<256>), dim3((unsigned)g->ids.size()), dim3(256), 0, s, g->d_insts,
+                       (int)g->ids.size(), g->d_counts, g->d_offsets, g->d_done, rotate);
+}
+
+hipEvent_t pooled_event(dba_ctx* ctx) {
+    if (!ctx->ev_pool.empty()) {
+        hipEvent_t e = ctx->ev_pool.back();
+        ctx->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+dba_status partition(dba_ctx* ctx, const int32_t* ids, int32_t n_ids, std::vector<std::vector<int>>* parts) {
+    if (n_ids <= 0 || !ids) return fail(DBA_ERR_INVALID, "no instances given");
+    std::vector<int> seen;
+    for (int k = 0; k < n_ids; k++) {
+        int id = ids[k];
+        auto it = ctx->inst.find(id);
+        if (it == ctx->inst.end()) return fail(DBA_ERR_NOTFOUND, "unknown instance %d", id);
+        if (std::find(seen.begin(), seen.end(), id) != seen.end())
+            return fail(DBA_ERR_INVALID, "instance %d listed twice", id);
+        seen.push_back(id);
+        const dba_params& p = it->second->p;
+        int thr = p.threads ? p.threads : default_threads(p.model, p.tile_x);
+        bool placed = false;
+        for (auto& part : *parts) {
+            const dba_params& q = ctx->inst[part[0]]->p;
+            int thq = q.threads ? q.threads : default_threads(q.model, q.tile_x);
+            if (q.model == p.model && q.tile_x == p.tile_x && thq == thr && part.size() < 255) {
+                part.push_back(id);
+                placed = true;
+                break;
+            }
+        }
+        if (!placed) parts->push_back({id});
+    }
+    return DBA_OK;
+}
+
+dba_status enqueue_steps(dba_ctx* ctx, const std::vector<Group*>& gs, int n_steps, bool timed) {
+    for (int s = 0; s < n_steps; s++) {
+        for (Group* g : gs) {
+            hipEvent_t a = nullptr, b = nullptr, c = nullptr;
+            if (timed) {
+                a = pooled_event(ctx);
+                b = pooled_event(ctx);
+                c = pooled_event(ctx);
+                (void)hipEventRecord(a, ctx->stream);
+            }
+            launch_step(g, ctx->stream);
+            LAUNCH_CHECK();
+            if (timed) (void)hipEventRecord(b, ctx->stream);
+            launch_finalize(g, ctx->stream, 1);
+            LAUNCH_CHECK();
+            if (timed) {
+                (void)hipEventRecord(c, ctx->stream);
+                ctx->ev_step.push_back({a, b});
+                ctx->ev_fin.push_back({b, c});
+            }
+        }
+    }
+    return DBA_OK;
+}
+
+dba_status prepare_groups(dba_ctx* ctx, const int32_t* ids, int32_t n_ids, std::vector<Group*>* gs) {
+    std::vector<std::vector<int>> parts;
+    dba_status st = partition(ctx, ids, n_ids, &parts);
+    if (st != DBA_OK) return st;
+    for (auto& part : parts) {
+        Group* g = nullptr;
+        st = get_group(ctx, part, &g);
+        if (st != DBA_OK) return st;
+        gs->push_back(g);
+    }
+    // (re)build the worklists from the current awake sets
+    for (Group* g : *gs) {
+        launch_finalize(g, ctx->stream, 0);
+        LAUNCH_CHECK();
+    }
+    return DBA_OK;
+}
+
+Instance* find(dba_ctx* ctx, int id) {
+    auto it = ctx->inst.find(id);
+    return it == ctx->inst.end() ? nullptr : it->second;
+}
+
+dba_status ensure_staging(dba_ctx* ctx, size_t n) {
+    if (ctx->staging_n >= n) return DBA_OK;
+    if (ctx->staging) (void)hipFree(ctx->staging);
+    ctx->staging = nullptr;
+    ctx->staging_n = 0;
+    HIP_TRY(hipMalloc(&ctx->staging, n * sizeof(double)));
+    ctx->staging_n = n;
+    return DBA_OK;
+}
+
+}  // namespace
+
+// ===================================================================== C ABI
+
+extern "C" {
+
+const char* dba_last_error(void) { return g_error.c_str(); }
+
+dba_status dba_init(int32_t device, dba_ctx** out) {
+    if (!out) return fail(DBA_ERR_INVALID, "out is null");
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(DBA_ERR_INVALID, "device %d not present (%d devices)", device, n);
+    HIP_TRY(hipSetDevice(device));
+    dba_ctx* ctx = new dba_ctx();
+    ctx->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return fail(DBA_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
+    }
+    HIP_TRY(hipMalloc(&ctx->d_flag, sizeof(int)));
+    for (auto& m : ctx->markers) HIP_TRY(hipEventCreate(&m));
+    *out = ctx;
+    return DBA_OK;
+}
+
+dba_status dba_finalize(dba_ctx* ctx) {
+    if (!ctx) return DBA_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    std::vector<int> ids;
+    for (auto& kv : ctx->inst) ids.push_back(kv.first);
+    for (int id : ids) dba_instance_destroy(ctx, id);
+    for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto& e : ctx->ev_step) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    for (auto& e : ctx->ev_fin) (void)hipEventDestroy(e.second);
+    for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
+    for (auto m : ctx->markers) (void)hipEventDestroy(m);
+    if (ctx->staging) (void)hipFree(ctx->staging);
+    if (ctx->d_flag) (void)hipFree(ctx->d_flag);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return DBA_OK;
+}
+
+dba_status dba_footprint(const dba_params* p, int64_t* bytes) {
+    if (!p || !bytes) return fail(DBA_ERR_INVALID, "null argument");
+    dba_status st = validate(*p);
+    if (st != DBA_OK) return st;
+    *bytes = footprint_of(*p);
+    return DBA_OK;
+}
+
+dba_status dba_instance_create(dba_ctx* ctx, const dba_params* pp, int32_t* id) {
+    if (!ctx || !pp || !id) return fail(DBA_ERR_INVALID, "null argument");
+    dba_status st = validate(*pp);
+    if (st != DBA_OK) return st;
+    HIP_TRY(hipSetDevice(ctx->device));
+    Instance* in = new Instance();
+    in->p = *pp;
+    if (in->p.tile_z <= 0) in->p.tile_z = 1;
+    const dba_params& p = in->p;
+    Layout L = layout_of(p);
+    Inst& h = in->h;
+    memset(&h, 0, sizeof(h));
+    h.model = p.model;
+    h.mode = p.mode;
+    h.nx = p.nx; h.ny = p.ny; h.nz = p.nz;
+    h.tx = p.tile_x; h.ty = p.tile_y; h.tz = p.tile_z;
+    h.ntx = L.ntx; h.nty = L.nty; h.ntz = L.ntz; h.nt = L.nt; h.nwords = L.nwords;
+    h.bc_x = p.bc_x; h.bc_y = p.bc_y; h.bc_z = p.bc_z;
+    h.stencil = p.stencil; h.aniso_j = p.aniso_j; h.dt_mode = p.dt_mode; h.metric_kind = p.metric_kind;
+    h.nsets = L.nsets;
+    h.nfields = L.nf;
+    h.ncons = (p.mode == DBA_MODE_CONSERVED) ? (p.model == DBA_MODEL_CH ? 1 : 0) : 0;
+    for (int f = 0; f < kMaxFields; f++) h.field_set[f] = 0;
+    if (L.nsets == 2) h.field_set[1] = 1;
+    h.full_tiles = (p.nx % p.tile_x == 0 && p.ny % p.tile_y == 0 && p.nz % p.tile_z == 0) ? 1 : 0;
+    h.dx = p.dx; h.dy = p.dy > 0 ? p.dy : p.dx; h.dz = p.dz > 0 ? p.dz : p.dx; h.dt = p.dt;
+    h.thresh_a = p.thresh_a; h.thresh_b = p.thresh_b;
+    h.ntotal = (double)L.n;
+    derive_constants(p, h);
+    h.tau = p.tau; h.eps_bar = p.eps_bar; h.delta = p.delta; h.alpha = p.alpha; h.gamma = p.gamma;
+    h.theta0 = p.theta0; h.latent_k = p.latent_k; h.t_eq = p.t_eq;
+    h.mobility = p.mobility; h.kappa = p.kappa; h.react_gamma = p.react_gamma; h.cbar = p.cbar;
+    h.chi_xy = p.chi_xy; h.chi_xz = p.chi_xz; h.chi_yz = p.chi_yz;
+    h.chi_xb = p.chi_xb; h.chi_yb = p.chi_yb; h.chi_zb = p.chi_zb;
+    h.v_x = p.v_x; h.v_y = p.v_y; h.v_z = p.v_z; h.r_x = p.r_x; h.r_y = p.r_y; h.r_z = p.r_z;
+    h.n_exp = p.n_exp; h.m_exp = p.m_exp;
+    h.epsx_sq = p.epsx_sq; h.epsy_sq = p.epsy_sq; h.epsz_sq = p.epsz_sq; h.epsphi_sq = p.epsphi_sq;
+    h.mob_x = p.mob_x; h.mob_y = p.mob_y; h.mob_z = p.mob_z; h.mob_phi = p.mob_phi;
+    h.k_eq = p.k_eq; h.z_crit = p.z_crit; h.p_gel = p.p_gel; h.k_0 = p.k_0;
+    h.gas_gamma = p.gas_gamma; h.cfl = p.cfl; h.vol = h.dx * h.dy;
+    h.hist_cap = std::max(0, p.history_cap);
+
+#define ALLOC(ptr, count)                         \
+    do {                                          \
+        dba_status s_ = dalloc(in, &(ptr), (count)); \
+        if (s_ != DBA_OK) { free_instance(in); delete in; return s_; } \
+    } while (0)
+    for (int f = 0; f < L.nf; f++) {
+        ALLOC(h.buf[0][f], (size_t)L.n);
+        ALLOC(h.buf[1][f], (size_t)L.n);
+    }
+    for (int a = 0; a < L.naux; a++) ALLOC(h.aux[a], (size_t)L.n);
+    for (int s = 0; s < 2; s++) {
+        ALLOC(h.a_cur[s], (size_t)L.nwords);
+        ALLOC(h.a_next[s], (size_t)L.nwords);
+        ALLOC(h.par[s], (size_t)L.nwords);
+    }
+    ALLOC(h.tpart, (size_t)kMaxFields * L.nt);
+    ALLOC(h.tspeed, (size_t)2 * L.nt);
+    ALLOC(h.s_tile, (size_t)kMaxFields * 2 * L.nt);
+    ALLOC(h.scal, 16);
+    ALLOC(h.ctr, 4);
+    ALLOC(h.hist, (size_t)2 * h.hist_cap + 2);
+    ALLOC(h.seg, (size_t)L.nt);
+    ALLOC(in->d_self, 1);
+#undef ALLOC
+    // all tiles awake at start (example_Dendritic_growth/kernel.cu:149-161)
+    std::vector<uint64_t> all((size_t)L.nwords, ~0ull);
+    if (L.nt % 64) all.back() = (1ull << (L.nt % 64)) - 1ull;
+    double scal[16] = {0};
+    scal[1] = p.dt;
+    for (int s = 0; s < L.nsets; s++)
+        HIP_TRY(hipMemcpy(h.a_cur[s], all.data(), sizeof(uint64_t) * L.nwords, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h.scal, scal, sizeof(scal), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(in->d_self, &h, sizeof(Inst), hipMemcpyHostToDevice));
+    // the null-stream memsets/copies above must land before work on ctx->stream
+    HIP_TRY(hipDeviceSynchronize());
+    in->footprint = footprint_of(p);
+    in->ncell = L.n;
+    in->id = ctx->next_id++;
+    ctx->inst[in->id] = in;
+    *id = in->id;
+    return DBA_OK;
+}
+
+dba_status dba_instance_destroy(dba_ctx* ctx, int32_t id) {
+    if (!ctx) return fail(DBA_ERR_INVALID, "null ctx");
+    Instance* in = find(ctx, id);
+    if (!in) return fail(DBA_ERR_NOTFOUND, "unknown instance %d", id);
+    (void)hipStreamSynchronize(ctx->stream);
+    invalidate_groups(ctx, id);
+    free_instance(in);
+    ctx->inst.erase(id);
+    delete in;
+    return DBA_OK;
+}
+
+dba_status dba_instance_info(dba_ctx* ctx, int32_t id, dba_info* out) {
+    if (!ctx || !out) return fail(DBA_ERR_INVALID, "null argument");
+    Instance* in = find(ctx, id);
+    if (!in) return fail(DBA_ERR_NOTFOUND, "unknown instance %d", id);
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    double scal[16];
+    int64_t ctr[4];
+    HIP_TRY(hipMemcpy(scal, in->h.scal, sizeof(scal), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(ctr, in->h.ctr, sizeof(ctr), hipMemcpyDeviceToHost));
+    out->steps_done = ctr[0];
+    out->cell_updates = ctr[1];
+    out->ntiles = in->h.nt;
+    out->ntiles_x = in->h.ntx;
+    out->ntiles_y = in->h.nty;
+    out->ntiles_z = in->h.ntz;
+    out->sim_time = scal[2];
+    out->last_dt = scal[3];
+    out->cons_offset = scal[8];
+    out->footprint_bytes = in->footprint;
+    return DBA_OK;
+}
+
+dba_status dba_upload(dba_ctx* ctx, int32_t id, int32_t field, const double* host, size_t n) {
+    if (!ctx || !host) return fail(DBA_ERR_INVALID, "null argument");
+    Instance* in = find(ctx, id);
+    if (!in) return fail(DBA_ERR_NOTFOUND, "unknown instance %d", id);
+    if (field < 0 || field >= in->h.nfields) return fail(DBA_ERR_INVALID, "field %d invalid", field);
+    if ((int64_t)n != in->ncell) return fail(DBA_ERR_INVALID, "n=%zu, expected %lld", n, (long long)in->ncell);
+    HIP_TRY(hipMemcpyAsync(in->h.buf[0][field], host, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(in->h.buf[1][field], in->h.buf[0][field], n * sizeof(double), hipMemcpyDeviceToDevice,
+                           ctx->stream));
+    if (in->h.ncons > 0) {
+        hipLaunchKernelGGL(reset_epochs_kernel, dim3(64), dim3(256), 0, ctx->stream, in->d_self, 0);
+        LAUNCH_CHECK();
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return DBA_OK;
+}
+
+dba_status dba_download(dba_ctx* ctx, int32_t id, int32_t field, double* host, size_t n) {
+    if (!ctx || !host) return fail(DBA_ERR_INVALID, "null argument");
+    Instance* in = find(ctx, id);
+    if (!in) return fail(DBA_ERR_NOTFOUND, "unknown instance %d", id);
+    if (field < 0 || field >= in->h.nfields) return fail(DBA_ERR_INVALID, "field %d invalid", field);
+    if ((int64_t)n != in->ncell) return fail(DBA_ERR_INVALID, "n=%zu, expected %lld", n, (long long)in->ncell);
+    dba_status st = ensure_staging(ctx, n);
+    if (st != DBA_OK) return st;
+    unsigned blocks = (unsigned)std::min<int64_t>(4096, (in->ncell + 255) / 256);
+    hipLaunchKernelGGL(materialize_kernel, dim3(blocks), dim3(256), 0, ctx->stream, in->d_self, 0, field,
+                       ctx->staging);
+    LAUNCH_CHECK();
+    HIP_TRY(hipMemcpyAsync(host, ctx->staging, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return DBA_OK;
+}
+
+dba_status dba_step(dba_ctx* ctx, const int32_t* ids, int32_t n_ids, int32_t n_steps) {
+    if (!ctx) return fail(DBA_ERR_INVALID, "null ctx");
+    if (n_steps < 0) return fail(DBA_ERR_INVALID, "n_steps < 0");
+    std::vector<Group*> gs;
+    dba_status st = prepare_groups(ctx, ids, n_ids, &gs);
+    if (st != DBA_OK) return st;
+    return enqueue_steps(ctx, gs, n_steps, ctx->timing);
+}
+
+dba_status dba_step_graph(dba_ctx* ctx, const int32_t* ids, int32_t n_ids, int32_t n_steps, int32_t chunk) {
+    if (!ctx) return fail(DBA_ERR_INVALID, "null ctx");
+    if (n_steps < 0 || chunk < 1) return fail(DBA_ERR_INVALID, "n_steps < 0 or chunk < 1");
+    std::vector<Group*> gs;
+    dba_status st = prepare_groups(ctx, ids, n_ids, &gs);
+    if (st != DBA_OK) return st;
+    int reps = n_steps / chunk, rem = n_steps % chunk;
+    if (reps > 0) {
+        GraphKey key{std::vector<int>(ids, ids + n_ids), chunk};
+        auto it = ctx->graphs.find(key);
+        hipGraphExec_t exec = nullptr;
+        if (it == ctx->graphs.end()) {
+            hipGraph_t graph = nullptr;
+            HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+            dba_status est = enqueue_steps(ctx, gs, chunk, false);
+            hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+            if (est != DBA_OK) return est;
+            if (e != hipSuccess) return fail(DBA_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+            HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(graph);
+            ctx->graphs[key] = exec;
+        } else {
+            exec = it->second;
+        }
+        for (int r = 0; r < reps; r++) HIP_TRY(hipGraphLaunch(exec, ctx->stream));
+    }
+    if (rem > 0) return enqueue_steps(ctx, gs, rem, false);
+    return DBA_OK;
+}
+
+dba_status dba_sync(dba_ctx* ctx) {
+    if (!ctx) return fail(DBA_ERR_INVALID, "null ctx");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return DBA_OK;
+}
+
+dba_status dba_active_tiles(dba_ctx* ctx, int32_t id, int32_t set, int32_t* count, uint8_t* mask) {
+    if (!ctx || !count) return fail(DBA_ERR_INVALID, "null argument");
+    Instance* in = find(ctx, id);
+    if (!in) return fail(DBA_ERR_NOTFOUND, "unknown instance %d", id);
+    if (set < 0 || set >= in->h.nsets) return fail(DBA_ERR_INVALID, "set %d invalid", set);
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    int nt = in->h.nt;
+    std::vector<uint64_t> w((size_t)in->h.nwords);
+    HIP_TRY(hipMemcpy(w.data(), in->h.a_cur[set], sizeof(uint64_t) * w.size(), hipMemcpyDeviceToHost));
+    int c = 0;
+    for (int t = 0; t < nt; t++) {
+        int b = in->p.mode == DBA_MODE_REGULAR ? 1 : (int)((w[t >> 6] >> (t & 63)) & 1ull);
+        c += b;
+        if (mask) mask[t] = (uint8_t)b;
+    }
+    *count = c;
+    return DBA_OK;
+}
+
+dba_status dba_worklist(dba_ctx* ctx, int32_t id, int32_t* tiles, int32_t cap, int32_t* count) {
+    if (!ctx || !count) return fail(DBA_ERR_INVALID, "null argument");
+    Instance* in = find(ctx, id);
+    if (!in) return fail(DBA_ERR_NOTFOUND, "unknown instance %d", id);
+    // compact the instance's current awake set (the list the next step runs)
+    std::vector<Group*> gs;
+    dba_status st = prepare_groups(ctx, &id, 1, &gs);
+    if (st != DBA_OK) return st;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    int64_t ctr[4];
+    HIP_TRY(hipMemcpy(ctr, in->h.ctr, sizeof(ctr), hipMemcpyDeviceToHost));
+    *count = (int32_t)ctr[2];
+    if (tiles && cap > 0) {
+        int n = (int)std::min<int64_t>(cap, ctr[2]);
+        if (n > 0) HIP_TRY(hipMemcpy(tiles, in->h.seg, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    }
+    return DBA_OK;
+}
+
+dba_status dba_active_history(dba_ctx* ctx, int32_t id, int32_t* counts, int32_t max_steps, int32_t* n_steps) {
+    if (!ctx || !n_steps) return fail(DBA_ERR_INVALID, "null argument");
+    Instance* in = find(ctx, id);
+    if (!in) return fail(DBA_ERR_NOTFOUND, "unknown instance %d", id);
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    int64_t ctr[4];
+    HIP_TRY(hipMemcpy(ctr, in->h.ctr, sizeof(ctr), hipMemcpyDeviceToHost));
+    int n = (int)std::min<int64_t>(std::min<int64_t>(ctr[0], in->h.hist_cap), max_steps);
+    if (counts && n > 0) HIP_TRY(hipMemcpy(counts, in->h.hist, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToHost));
+    *n_steps = n;
+    return DBA_OK;
+}
+
+dba_status dba_timing_enable(dba_ctx* ctx, int32_t enable) {
+    if (!ctx) return fail(DBA_ERR_INVALID, "null ctx");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (auto& e : ctx->ev_step) { ctx->ev_pool.push_back(e.first); ctx->ev_pool.push_back(e.second); }
+    for (auto& e : ctx->ev_fin) ctx->ev_pool.push_back(e.second);
+    ctx->ev_step.clear();
+    ctx->ev_fin.clear();
+    ctx->timing = enable != 0;
+    return DBA_OK;
+}
+
+dba_status dba_timing_query(dba_ctx* ctx, double* step_ms, int64_t* step_n, double* fin_ms, int64_t* fin_n) {
+    if (!ctx) return fail(DBA_ERR_INVALID, "null ctx");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    double a = 0.0, b = 0.0;
+    for (auto& e : ctx->ev_step) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e.first, e.second));
+        a += ms;
+    }
+    for (auto& e : ctx->ev_fin) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e.first, e.second));
+        b += ms;
+    }
+    if (step_ms) *step_ms = a;
+    if (step_n) *step_n = (int64_t)ctx->ev_step.size();
+    if (fin_ms) *fin_ms = b;
+    if (fin_n) *fin_n = (int64_t)ctx->ev_fin.size();
+    return DBA_OK;
+}
+
+dba_status dba_marker(dba_ctx* ctx, int32_t slot) {
+    if (!ctx || slot < 0 || slot >= 8) return fail(DBA_ERR_INVALID, "bad marker slot");
+    HIP_TRY(hipEventRecord(ctx->markers[slot], ctx->stream));
+    return DBA_OK;
+}
+
+dba_status dba_marker_elapsed(dba_ctx* ctx, int32_t a, int32_t b, double* ms) {
+    if (!ctx || !ms || a < 0 || a >= 8 || b < 0 || b >= 8) return fail(DBA_ERR_INVALID, "bad marker slot");
+    HIP_TRY(hipEventSynchronize(ctx->markers[b]));
+    float f = 0.f;
+    HIP_TRY(hipEventElapsedTime(&f, ctx->markers[a], ctx->markers[b]));
+    *ms = f;
+    return DBA_OK;
+}
+
+dba_status dba_check_finite(dba_ctx* ctx, int32_t id) {
+    if (!ctx) return fail(DBA_ERR_INVALID, "null ctx");
+    Instance* in = find(ctx, id);
+    if (!in) return fail(DBA_ERR_NOTFOUND, "unknown instance %d", id);
+    HIP_TRY(hipMemsetAsync(ctx->d_flag, 0, sizeof(int), ctx->stream));
+    unsigned blocks = (unsigned)std::min<int64_t>(2048, (in->ncell + 255) / 256);
+    hipLaunchKernelGGL(finite_kernel, dim3(blocks), dim3(256), 0, ctx->stream, in->d_self, 0, ctx->d_flag);
+    LAUNCH_CHECK();
+    int bad = 0;
+    HIP_TRY(hipMemcpyAsync(&bad, ctx->d_flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (bad) return fail(DBA_ERR_NUMERIC, "instance %d holds non-finite values", id);
+    return DBA_OK;
+}
+
+}  // extern "C"

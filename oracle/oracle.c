@@ -1,0 +1,521 @@
+/*
+ * oracle.c - CPU restatement of the reference DBA time step (TEST INFRASTRUCTURE).
+ *
+ * Not the product: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load it. See oracle.h for the reference files it follows.
+ *
+ * Built with -O2 -ffp-contract=off so every +,-,*,/ is one IEEE operation in
+ * source order, like the HIP kernels (also built with -ffp-contract=off).
+ *
+ * Parity pinning status per function (the checks are in tests/test_oracle_*.py):
+ *   ch_*      pinned: 5/9-point Laplacian Fourier symbols, exact discrete mean law,
+ *             single-mode linear growth factor, conserved-DBA total, fixed points.
+ *   k93_*     pinned: fixed points, frozen-phi heat-equation modes, hand-computed
+ *             single-cell update, anisotropy vs atan2/cos/sin, D4 symmetry.
+ *   dilate / compact  pinned: brute force over all small flag patterns.
+ */
+#define _GNU_SOURCE
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include "oracle.h"
+
+#define MAXF 4
+#define MAXW 32
+
+struct ora_sim {
+    dba_params p;
+    int nx, ny, nz;
+    long n;
+    int tx, ty, tz, ntx, nty, ntz, nt;
+    double* f[MAXF];    /* current fields */
+    double* fn[MAXF];   /* new fields */
+    double* w[MAXW];    /* derivative / derived arrays */
+    uint8_t* awake[2];
+    uint8_t* next[2];
+    int32_t* hist;
+    int hist_n, hist_cap;
+    long steps;
+    double time, last_dt;
+    /* derived constants (identical expressions in the HIP host code) */
+    double invdx2, inv6dx2, inv2dx, dt_tau, alpha_pi, cos_jt0, sin_jt0, k_epsp;
+};
+
+/* ------------------------------------------------------------------ helpers */
+
+static inline int map_i(int i, int n, int bc) {
+    if (bc == DBA_BC_PERIODIC) {
+        i %= n;
+        if (i < 0) i += n;
+        return i;
+    }
+    if (i < 0) return -1 - i;
+    if (i >= n) return 2 * n - 1 - i;
+    return i;
+}
+
+static inline long idx2(const ora_sim* s, int x, int y) {
+    return (long)map_i(x, s->nx, s->p.bc_x) + (long)map_i(y, s->ny, s->p.bc_y) * s->nx;
+}
+
+static inline int tile_of(const ora_sim* s, int x, int y, int z) {
+    return x / s->tx + (y / s->ty) * s->ntx + (z / s->tz) * s->ntx * s->nty;
+}
+
+static inline int active(const ora_sim* s, int set, int x, int y, int z) {
+    if (s->p.mode == DBA_MODE_REGULAR) return 1;
+    return s->awake[set][tile_of(s, x, y, z)];
+}
+
+/* 5-point and 9-point (isotropic, the 27-point stencil of
+   example_Reactive_ModelB/kernel.cu:290-347 reduced to one z-plane) Laplacians. */
+static inline double lap5(const ora_sim* s, const double* f, int x, int y) {
+    double s1 = (f[idx2(s, x + 1, y)] + f[idx2(s, x - 1, y)]) + (f[idx2(s, x, y + 1)] + f[idx2(s, x, y - 1)]);
+    return (s1 - 4.0 * f[idx2(s, x, y)]) * s->invdx2;
+}
+static inline double lap9(const ora_sim* s, const double* f, int x, int y) {
+    double s1 = (f[idx2(s, x + 1, y)] + f[idx2(s, x - 1, y)]) + (f[idx2(s, x, y + 1)] + f[idx2(s, x, y - 1)]);
+    double s2 = (f[idx2(s, x + 1, y + 1)] + f[idx2(s, x - 1, y + 1)]) + (f[idx2(s, x + 1, y - 1)] + f[idx2(s, x - 1, y - 1)]);
+    return ((4.0 * s1 + s2) - 20.0 * f[idx2(s, x, y)]) * s->inv6dx2;
+}
+
+void ora_dilate(const uint8_t* N, uint8_t* A, int ntx, int nty, int ntz, int kind,
+                int wrap_x, int wrap_y, int wrap_z) {
+    /* example_Dendritic_growth/kernel.cu:577-634 (faces, periodic),
+       example_Kelvin_Helmholtz/KH_DBA.cu:988-1044 (3x3 incl. diagonals + z faces). */
+    long nt = (long)ntx * nty * ntz;
+    memset(A, 0, (size_t)nt);
+    for (int k = 0; k < ntz; k++)
+        for (int j = 0; j < nty; j++)
+            for (int i = 0; i < ntx; i++) {
+                long t = i + (long)j * ntx + (long)k * ntx * nty;
+                if (!N[t]) continue;
+                for (int dk = -1; dk <= 1; dk++)
+                    for (int dj = -1; dj <= 1; dj++)
+                        for (int di = -1; di <= 1; di++) {
+                            int nz_ = (di != 0) + (dj != 0) + (dk != 0);
+                            if (kind == 0 && nz_ > 1) continue;
+                            if (kind == 1 && dk != 0 && (di != 0 || dj != 0)) continue;
+                            int ii = i + di, jj = j + dj, kk = k + dk;
+                            if (ii < 0 || ii >= ntx) { if (!wrap_x) continue; ii = (ii + ntx) % ntx; }
+                            if (jj < 0 || jj >= nty) { if (!wrap_y) continue; jj = (jj + nty) % nty; }
+                            if (kk < 0 || kk >= ntz) { if (!wrap_z) continue; kk = (kk + ntz) % ntz; }
+                            A[ii + (long)jj * ntx + (long)kk * ntx * nty] = 1;
+                        }
+            }
+}
+
+int ora_compact(const uint8_t* A, int nt, int32_t* out) {
+    int c = 0;
+    for (int t = 0; t < nt; t++)
+        if (A[t]) out[c++] = t;
+    return c;
+}
+
+static void record_history(ora_sim* s) {
+    if (s->hist_n >= s->hist_cap) return;
+    int c0 = 0, c1 = 0;
+    if (s->p.mode == DBA_MODE_REGULAR) {
+        c0 = c1 = s->nt;
+    } else {
+        for (int t = 0; t < s->nt; t++) { c0 += s->awake[0][t]; c1 += s->awake[1][t]; }
+    }
+    s->hist[2 * s->hist_n] = c0;
+    s->hist[2 * s->hist_n + 1] = c1;
+    s->hist_n++;
+}
+
+/* Wakeup_Next + Wakeup_Neighbor for one set, given per-cell criterion flags
+   already folded into next[] (example_Reactive_ModelB/kernel.cu:657-763). */
+static void dilate_set(ora_sim* s, int set) {
+    int wx = s->p.bc_x == DBA_BC_PERIODIC, wy = s->p.bc_y == DBA_BC_PERIODIC, wz = s->p.bc_z == DBA_BC_PERIODIC;
+    int kind = s->p.model == DBA_MODEL_KH ? 1 : 0;
+    ora_dilate(s->next[set], s->awake[set], s->ntx, s->nty, s->ntz, kind, wx, wy, wz);
+}
+
+/* --------------------------------------------------------------- CH model */
+/* dc/dt = M lap(c^3 - c - kappa lap c) - Gamma (c - cbar), 2D.
+   Loop structure of example_Reactive_ModelB/CPU_version/base_CPU.cpp:563-596:
+   Calc_mu (all cells) -> Update -> Swap; DBA semantics of
+   example_Reactive_ModelB/kernel.cu:1094-1131 (mu of sleeping blocks stays stale).
+   w[0] = mu, w[1] = metric |lap c|, w[2] = change (conserved). */
+
+static inline double ch_lap(const ora_sim* s, const double* f, int x, int y) {
+    return s->p.stencil == 5 ? lap5(s, f, x, y) : lap9(s, f, x, y);
+}
+
+static void ch_calc_mu(ora_sim* s, int y0, int y1) {
+    const double* c = s->f[0];
+    double* mu = s->w[0];
+    double* metric = s->w[1];
+    for (int y = y0; y < y1; y++)
+        for (int x = 0; x < s->nx; x++) {
+            if (!active(s, 0, x, y, 0)) continue;
+            long i = (long)x + (long)y * s->nx;
+            double lc = ch_lap(s, c, x, y);
+            double ci = c[i];
+            mu[i] = (ci * ci * ci - ci) - s->p.kappa * lc;
+            metric[i] = fabs(lc);
+        }
+}
+
+static void ch_update(ora_sim* s, int y0, int y1) {
+    const double* c = s->f[0];
+    double* cn = s->fn[0];
+    const double* mu = s->w[0];
+    double* chg = s->w[2];
+    int cons = s->p.mode == DBA_MODE_CONSERVED;
+    for (int y = y0; y < y1; y++)
+        for (int x = 0; x < s->nx; x++) {
+            long i = (long)x + (long)y * s->nx;
+            if (!active(s, 0, x, y, 0)) {
+                if (cons) chg[i] = 0.0;  /* example_Reactive_ModelB/Conserved/kernel.cu:688-700 */
+                continue;
+            }
+            double lm = ch_lap(s, mu, x, y);
+            double change = s->p.dt * (s->p.mobility * lm - s->p.react_gamma * (c[i] - s->p.cbar));
+            cn[i] = c[i] + change;
+            if (cons) chg[i] = change;
+        }
+}
+
+static void ch_step(ora_sim* s) {
+    long n = s->n;
+    ch_calc_mu(s, 0, s->ny);
+    ch_update(s, 0, s->ny);
+    if (s->p.mode == DBA_MODE_REGULAR) {
+        memcpy(s->f[0], s->fn[0], sizeof(double) * n);
+        return;
+    }
+    if (s->p.mode == DBA_MODE_CONSERVED) {
+        /* thrust::reduce of change / N_total, subtracted from every cell, then a
+           full swap (example_Reactive_ModelB/Conserved/kernel.cu:1152-1173). */
+        double sum = 0.0;
+        for (long i = 0; i < n; i++) sum += s->w[2][i];
+        double avg = sum / (double)n;
+        for (long i = 0; i < n; i++) s->fn[0][i] -= avg;
+        memcpy(s->f[0], s->fn[0], sizeof(double) * n);
+    } else {
+        for (int y = 0; y < s->ny; y++)
+            for (int x = 0; x < s->nx; x++)
+                if (active(s, 0, x, y, 0)) {
+                    long i = (long)x + (long)y * s->nx;
+                    s->f[0][i] = s->fn[0][i];
+                }
+    }
+    /* Wakeup_Next: any(metric > Metric_eps) over awake blocks */
+    memset(s->next[0], 0, (size_t)s->nt);
+    for (int y = 0; y < s->ny; y++)
+        for (int x = 0; x < s->nx; x++) {
+            int t = tile_of(s, x, y, 0);
+            if (s->awake[0][t] && s->w[1][(long)x + (long)y * s->nx] > s->p.thresh_a) s->next[0][t] = 1;
+        }
+    dilate_set(s, 0);
+}
+
+/* -------------------------------------------------------- Kobayashi-93 model */
+/* tau dphi/dt = -d/dx(eps eps' phi_y) + d/dy(eps eps' phi_x) + eps^2 lap phi
+                 + phi(1-phi)(phi - 1/2 + m),  m = (alpha/pi) atan(gamma (T_eq - T)),
+   dT/dt = lap T + K dphi/dt,  eps = eps_bar (1 + delta cos j(theta - theta0)),
+   theta = angle of grad phi. The angle enters only through cos/sin of j*theta,
+   evaluated as the j-th power of (phi_x + i phi_y)/|grad phi| (theta = 0 where
+   grad phi = 0, as atan2(0,0) = 0).
+   DBA semantics follow the reference dendrite (example_Dendritic_growth/kernel.cu:322-634):
+   separate p and T sets, T update K dphi only where the block is p-awake, latent
+   heat dropped in p-awake/T-asleep blocks.
+   w[0] = Fx = eps eps' phi_x, w[1] = Fy (both on a 1-cell ghost frame),
+   w[2] = eps^2, w[3] = lap phi, w[4] = lap T, w[5] = metric_T. */
+
+static inline void k93_aniso(const ora_sim* s, double gx, double gy, double* eps, double* a) {
+    double g2 = gx * gx + gy * gy;
+    double cj = 1.0, sj = 0.0;
+    if (g2 != 0.0) {
+        double r = sqrt(g2);
+        double u = gx / r, v = gy / r;
+        double cr = u, ci = v;
+        for (int k = 1; k < s->p.aniso_j; k++) {
+            double t = cr * u - ci * v;
+            ci = cr * v + ci * u;
+            cr = t;
+        }
+        cj = cr;
+        sj = ci;
+    }
+    double ca = cj * s->cos_jt0 + sj * s->sin_jt0;
+    double sa = sj * s->cos_jt0 - cj * s->sin_jt0;
+    double e = s->p.eps_bar * (1.0 + s->p.delta * ca);
+    double ep = s->k_epsp * sa;
+    *eps = e;
+    *a = e * ep;
+}
+
+static inline long fidx(const ora_sim* s, int x, int y) { /* ghost-frame index, x,y in [-1, n] */
+    return (long)(x + 1) + (long)(y + 1) * (s->nx + 2);
+}
+
+static void k93_finite_diff(ora_sim* s, int y0, int y1) {
+    const double* p = s->f[0];
+    const double* T = s->f[1];
+    /* Fx, Fy on rows [y0-1, y1] when the slab touches the frame */
+    int ya = y0 == 0 ? -1 : y0, yb = y1 == s->ny ? s->ny + 1 : y1;
+    for (int y = ya; y < yb; y++)
+        for (int x = -1; x <= s->nx; x++) {
+            double gx = (p[idx2(s, x + 1, y)] - p[idx2(s, x - 1, y)]) * s->inv2dx;
+            double gy = (p[idx2(s, x, y + 1)] - p[idx2(s, x, y - 1)]) * s->inv2dx;
+            double e, a;
+            k93_aniso(s, gx, gy, &e, &a);
+            long fi = fidx(s, x, y);
+            s->w[0][fi] = a * gx;
+            s->w[1][fi] = a * gy;
+            if (x >= 0 && x < s->nx && y >= 0 && y < s->ny) {
+                long i = (long)x + (long)y * s->nx;
+                s->w[2][i] = e * e;
+                s->w[3][i] = lap5(s, p, x, y);
+                s->w[4][i] = lap5(s, T, x, y);
+            }
+        }
+}
+
+static void k93_evolve(ora_sim* s, int y0, int y1) {
+    const double* p = s->f[0];
+    const double* T = s->f[1];
+    double* pn = s->fn[0];
+    double* Tn = s->fn[1];
+    const double* Fx = s->w[0];
+    const double* Fy = s->w[1];
+    for (int y = y0; y < y1; y++)
+        for (int x = 0; x < s->nx; x++) {
+            long i = (long)x + (long)y * s->nx;
+            int ap = active(s, 0, x, y, 0), at = active(s, 1, x, y, 0);
+            double pi_ = p[i];
+            if (ap) {
+                double cross = ((Fx[fidx(s, x, y + 1)] - Fx[fidx(s, x, y - 1)]) -
+                                (Fy[fidx(s, x + 1, y)] - Fy[fidx(s, x - 1, y)])) * s->inv2dx;
+                double m = s->alpha_pi * atan(s->p.gamma * (s->p.t_eq - T[i]));
+                double rhs = (cross + s->w[2][i] * s->w[3][i]) + (pi_ * (1.0 - pi_)) * ((pi_ - 0.5) + m);
+                pn[i] = pi_ + s->dt_tau * rhs;
+            }
+            if (at) {
+                double t1 = T[i] + s->p.dt * s->w[4][i];
+                Tn[i] = ap ? t1 + s->p.latent_k * (pn[i] - pi_) : t1;
+                s->w[5][i] = fabs(Tn[i] - T[i]);
+            }
+        }
+}
+
+static void k93_step(ora_sim* s) {
+    long n = s->n;
+    k93_finite_diff(s, 0, s->ny);
+    k93_evolve(s, 0, s->ny);
+    if (s->p.mode == DBA_MODE_REGULAR) {
+        memcpy(s->f[0], s->fn[0], sizeof(double) * n);
+        memcpy(s->f[1], s->fn[1], sizeof(double) * n);
+        return;
+    }
+    /* Wakeup_Next (example_Dendritic_growth/kernel.cu:496-574) */
+    memset(s->next[0], 0, (size_t)s->nt);
+    memset(s->next[1], 0, (size_t)s->nt);
+    double mp = s->p.thresh_a, mT = s->p.thresh_b;
+    for (int y = 0; y < s->ny; y++)
+        for (int x = 0; x < s->nx; x++) {
+            long i = (long)x + (long)y * s->nx;
+            int t = tile_of(s, x, y, 0);
+            if (s->awake[0][t]) {
+                s->f[0][i] = s->fn[0][i];
+                if (s->fn[0][i] >= mp && s->fn[0][i] <= (1.0 - mp)) s->next[0][t] = 1;
+            }
+            if (s->awake[1][t]) {
+                s->f[1][i] = s->fn[1][i];
+                if (s->w[5][i] >= mT) s->next[1][t] = 1;
+            }
+        }
+    dilate_set(s, 0);
+    dilate_set(s, 1);
+}
+
+/* ------------------------------------------------------------------ public */
+
+static void derive_constants(ora_sim* s) {
+    const dba_params* p = &s->p;
+    s->invdx2 = 1.0 / (p->dx * p->dx);
+    s->inv6dx2 = 1.0 / (6.0 * p->dx * p->dx);
+    s->inv2dx = 1.0 / (2.0 * p->dx);
+    s->dt_tau = p->tau != 0.0 ? p->dt / p->tau : 0.0;
+    s->alpha_pi = p->alpha / M_PI;
+    double jf = (double)p->aniso_j;
+    s->cos_jt0 = cos(jf * p->theta0);
+    s->sin_jt0 = sin(jf * p->theta0);
+    s->k_epsp = -(p->eps_bar * jf * p->delta);
+}
+
+static int nfields(int model) {
+    switch (model) {
+        case DBA_MODEL_CH: return 1;
+        case DBA_MODEL_DENDRITE_K93: return 2;
+        default: return 0;
+    }
+}
+
+ora_sim* ora_create(const dba_params* p) {
+    int nf = nfields(p->model);
+    if (nf == 0) return NULL;
+    if (p->nx <= 0 || p->ny <= 0 || p->nz != 1) return NULL;
+    ora_sim* s = (ora_sim*)calloc(1, sizeof(ora_sim));
+    s->p = *p;
+    s->nx = p->nx; s->ny = p->ny; s->nz = p->nz;
+    s->n = (long)s->nx * s->ny * s->nz;
+    s->tx = p->tile_x; s->ty = p->tile_y; s->tz = p->tile_z > 0 ? p->tile_z : 1;
+    s->ntx = (s->nx + s->tx - 1) / s->tx;
+    s->nty = (s->ny + s->ty - 1) / s->ty;
+    s->ntz = (s->nz + s->tz - 1) / s->tz;
+    s->nt = s->ntx * s->nty * s->ntz;
+    derive_constants(s);
+    for (int k = 0; k < nf; k++) {
+        s->f[k] = (double*)calloc((size_t)s->n, sizeof(double));
+        s->fn[k] = (double*)calloc((size_t)s->n, sizeof(double));
+    }
+    long nframe = (long)(s->nx + 2) * (s->ny + 2);
+    for (int k = 0; k < 8; k++) s->w[k] = (double*)calloc((size_t)(nframe > s->n ? nframe : s->n), sizeof(double));
+    for (int k = 0; k < 2; k++) {
+        s->awake[k] = (uint8_t*)malloc((size_t)s->nt);
+        memset(s->awake[k], 1, (size_t)s->nt);  /* all blocks awake at start */
+        s->next[k] = (uint8_t*)calloc((size_t)s->nt, 1);
+    }
+    s->hist_cap = p->history_cap > 0 ? p->history_cap : 0;
+    s->hist = (int32_t*)calloc((size_t)(2 * s->hist_cap + 2), sizeof(int32_t));
+    return s;
+}
+
+void ora_destroy(ora_sim* s) {
+    if (!s) return;
+    for (int k = 0; k < MAXF; k++) { free(s->f[k]); free(s->fn[k]); }
+    for (int k = 0; k < MAXW; k++) free(s->w[k]);
+    for (int k = 0; k < 2; k++) { free(s->awake[k]); free(s->next[k]); }
+    free(s->hist);
+    free(s);
+}
+
+int ora_set_field(ora_sim* s, int field, const double* src) {
+    if (field < 0 || field >= nfields(s->p.model)) return -1;
+    memcpy(s->f[field], src, sizeof(double) * s->n);
+    memcpy(s->fn[field], src, sizeof(double) * s->n);
+    return 0;
+}
+
+int ora_get_field(ora_sim* s, int field, double* dst) {
+    if (field < 0 || field >= nfields(s->p.model)) return -1;
+    memcpy(dst, s->f[field], sizeof(double) * s->n);
+    return 0;
+}
+
+int ora_step(ora_sim* s, int nsteps) {
+    for (int k = 0; k < nsteps; k++) {
+        record_history(s);
+        switch (s->p.model) {
+            case DBA_MODEL_CH: ch_step(s); break;
+            case DBA_MODEL_DENDRITE_K93: k93_step(s); break;
+            default: return -1;
+        }
+        s->steps++;
+        s->last_dt = s->p.dt;
+        s->time += s->p.dt;
+    }
+    return 0;
+}
+
+int ora_get_awake(ora_sim* s, int set, uint8_t* dst) {
+    if (set < 0 || set > 1) return -1;
+    if (s->p.mode == DBA_MODE_REGULAR) memset(dst, 1, (size_t)s->nt);
+    else memcpy(dst, s->awake[set], (size_t)s->nt);
+    return 0;
+}
+
+int ora_set_awake(ora_sim* s, int set, const uint8_t* src) {
+    if (set < 0 || set > 1) return -1;
+    memcpy(s->awake[set], src, (size_t)s->nt);
+    return 0;
+}
+
+int ora_get_next(ora_sim* s, int set, uint8_t* dst) {
+    if (set < 0 || set > 1) return -1;
+    memcpy(dst, s->next[set], (size_t)s->nt);
+    return 0;
+}
+
+int ora_ntiles(ora_sim* s) { return s->nt; }
+double ora_time(ora_sim* s) { return s->time; }
+double ora_last_dt(ora_sim* s) { return s->last_dt; }
+
+int ora_history(ora_sim* s, int32_t* out, int cap) {
+    int n = s->hist_n < cap ? s->hist_n : cap;
+    memcpy(out, s->hist, sizeof(int32_t) * 2 * (size_t)n);
+    return n;
+}
+
+/* -------------------------------------------- threaded regular-mode baseline */
+/* y-slab decomposition over pthreads with a barrier between phases, the
+   shared-memory analogue of the reference MPI_version z/y slabs
+   (example_Reactive_ModelB/CPU_version/MPI_version/base_CPU.cpp:182-226). */
+
+typedef struct {
+    ora_sim* s;
+    int y0, y1, nsteps;
+    pthread_barrier_t* bar;
+} slab_arg;
+
+static void slab_copy(ora_sim* s, int nf, int y0, int y1) {
+    for (int k = 0; k < nf; k++)
+        memcpy(s->f[k] + (long)y0 * s->nx, s->fn[k] + (long)y0 * s->nx, sizeof(double) * (size_t)(y1 - y0) * s->nx);
+}
+
+static void* slab_worker(void* argp) {
+    slab_arg* a = (slab_arg*)argp;
+    ora_sim* s = a->s;
+    for (int t = 0; t < a->nsteps; t++) {
+        if (s->p.model == DBA_MODEL_CH) {
+            ch_calc_mu(s, a->y0, a->y1);
+            pthread_barrier_wait(a->bar);
+            ch_update(s, a->y0, a->y1);
+            pthread_barrier_wait(a->bar);
+            slab_copy(s, 1, a->y0, a->y1);
+        } else {
+            k93_finite_diff(s, a->y0, a->y1);
+            pthread_barrier_wait(a->bar);
+            k93_evolve(s, a->y0, a->y1);
+            pthread_barrier_wait(a->bar);
+            slab_copy(s, 2, a->y0, a->y1);
+        }
+        pthread_barrier_wait(a->bar);
+    }
+    return NULL;
+}
+
+double ora_bench(ora_sim* s, int nsteps, int nthreads) {
+    if (s->p.mode != DBA_MODE_REGULAR) return -1.0;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > s->ny) nthreads = s->ny;
+    struct timespec t0, t1;
+    pthread_barrier_t bar;
+    pthread_barrier_init(&bar, NULL, (unsigned)nthreads);
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)nthreads);
+    slab_arg* args = (slab_arg*)malloc(sizeof(slab_arg) * (size_t)nthreads);
+    for (int k = 0; k < nthreads; k++) {
+        args[k].s = s;
+        args[k].y0 = (int)((long)s->ny * k / nthreads);
+        args[k].y1 = (int)((long)s->ny * (k + 1) / nthreads);
+        args[k].nsteps = nsteps;
+        args[k].bar = &bar;
+    }
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (int k = 1; k < nthreads; k++) pthread_create(&th[k], NULL, slab_worker, &args[k]);
+    slab_worker(&args[0]);
+    for (int k = 1; k < nthreads; k++) pthread_join(th[k], NULL);
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    pthread_barrier_destroy(&bar);
+    free(th);
+    free(args);
+    s->steps += nsteps;
+    return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}
