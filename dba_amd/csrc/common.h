@@ -15,7 +15,7 @@ struct Inst {
     double* buf[2][kMaxFields];  // double-buffered prognostic fields (tile parity picks one)
     double* aux[kMaxAux];        // model state kept for sleeping-neighbour semantics
     uint64_t* a_cur[2];          // awake bitmap of the step being executed, per set
-    uint64_t* a_next[2];         // awake bitmap of the next step (built by dilation atomics)
+    uint8_t* nflag[2];           // detector output per tile (Wakeup_Next), per set
     uint64_t* par[2];            // per-tile buffer parity per set: bit=1 -> buf[1] is current
     double* tpart;               // conserved: per-tile change sums, [field][nt]
     double* tspeed;              // KH: per-tile max wave speed, [0..nt) fresh (current U), [nt..2nt) stale
@@ -68,29 +68,6 @@ __device__ __forceinline__ int bit_of(const uint64_t* bm, int t) {
     return (int)((bm[t >> 6] >> (t & 63)) & 1ull);
 }
 
-// Set tile t and its neighbours in the next-step bitmap (Wakeup_Neighbor).
-// kind 0: face neighbours (4 in 2D, 6 in 3D); kind 1: 3x3 in-plane + z faces.
-__device__ __forceinline__ void dilate_into(uint64_t* bm, const Inst& I, int t, int kind) {
-    int ti = t % I.ntx;
-    int tj = (t / I.ntx) % I.nty;
-    int tk = t / (I.ntx * I.nty);
-    for (int dk = -1; dk <= 1; dk++) {
-        for (int dj = -1; dj <= 1; dj++) {
-            for (int di = -1; di <= 1; di++) {
-                int nzc = (di != 0) + (dj != 0) + (dk != 0);
-                if (kind == 0 && nzc > 1) continue;
-                if (kind == 1 && dk != 0 && (di != 0 || dj != 0)) continue;
-                int ii = ti + di, jj = tj + dj, kk = tk + dk;
-                if (ii < 0 || ii >= I.ntx) { if (I.bc_x != 0) continue; ii = (ii + I.ntx) % I.ntx; }
-                if (jj < 0 || jj >= I.nty) { if (I.bc_y != 0) continue; jj = (jj + I.nty) % I.nty; }
-                if (kk < 0 || kk >= I.ntz) { if (I.bc_z != 0) continue; kk = (kk + I.ntz) % I.ntz; }
-                int tt = ii + jj * I.ntx + kk * I.ntx * I.nty;
-                atomicOr((unsigned long long*)&bm[tt >> 6], 1ull << (tt & 63));
-            }
-        }
-    }
-}
-
 // Workgroup-wide OR of a per-thread predicate (wave ballot, then LDS across waves).
 template <int NT>
 __device__ __forceinline__ int block_any(int pred, int* lds_scratch) {
@@ -127,6 +104,57 @@ __device__ __forceinline__ double block_sum(double v, double* lds_scratch) {
     return r;
 }
 
+template <int W>
+__device__ __forceinline__ double lds_lap5(const double* s, int x, int y, double invdx2) {
+    double s1 = (s[(x + 1) + y * W] + s[(x - 1) + y * W]) + (s[x + (y + 1) * W] + s[x + (y - 1) * W]);
+    return (s1 - 4.0 * s[x + y * W]) * invdx2;
+}
+
+template <int W>
+__device__ __forceinline__ double lds_lap9(const double* s, int x, int y, double inv6dx2) {
+    double s1 = (s[(x + 1) + y * W] + s[(x - 1) + y * W]) + (s[x + (y + 1) * W] + s[x + (y - 1) * W]);
+    double s2 = (s[(x + 1) + (y + 1) * W] + s[(x - 1) + (y + 1) * W]) +
+                (s[(x + 1) + (y - 1) * W] + s[(x - 1) + (y - 1) * W]);
+    return ((4.0 * s1 + s2) - 20.0 * s[x + y * W]) * inv6dx2;
+}
+
+// The 3x3 block of tiles around tile t (2D), resolved once per tile into LDS:
+// for each direction d = (dx+1) + 3 (dy+1), the tile that owns the halo cells
+// on that side after the boundary map (wrap, or mirror back into t), its
+// buffer parity per set, whether it runs this step, and its conserved shift.
+struct NbTable {
+    int tile[9];
+    int par[2][9];
+    int act[2][9];
+    double shift[9];
+};
+
+// Fill by threads 0..8; caller synchronises.
+template <int TB>
+__device__ __forceinline__ void nb_fill(NbTable& nb, const Inst& I, int t, int x0, int y0, bool regular,
+                                        int nsets, bool cons, int consf) {
+    const int d = threadIdx.x;
+    if (d >= 9) return;
+    const int dx = d % 3 - 1, dy = d / 3 - 1;
+    const int rx = dx < 0 ? x0 - 1 : (dx > 0 ? x0 + TB : x0);
+    const int ry = dy < 0 ? y0 - 1 : (dy > 0 ? y0 + TB : y0);
+    const int mx = map_index(rx, I.nx, I.bc_x), my = map_index(ry, I.ny, I.bc_y);
+    const int tt = mx / TB + (my / TB) * I.ntx;
+    nb.tile[d] = tt;
+    for (int s = 0; s < nsets; s++) {
+        nb.par[s][d] = bit_of(I.par[s], tt);
+        nb.act[s][d] = regular ? 1 : bit_of(I.a_cur[s], tt);
+    }
+    nb.shift[d] = cons ? (I.scal[8 + consf] - I.s_tile[((int64_t)consf * 2 + nb.par[0][d]) * I.nt + tt]) : 0.0;
+}
+
+// Direction index of a cell at local coordinate l (0..W-1) of a region whose
+// interior [h, h+TB) is the tile.
+template <int TB, int H>
+__device__ __forceinline__ int nb_dir1(int l) {
+    return l < H ? 0 : (l < TB + H ? 1 : 2);
+}
+
 template <int NT>
 __device__ __forceinline__ double block_max(double v, double* lds_scratch) {
     constexpr int NW = NT / 64;
@@ -149,7 +177,31 @@ __device__ __forceinline__ int xcd_remap(int b, int G) {
     return (b & 7) * (G >> 3) + (b >> 3);
 }
 
-// Worklist item -> instance slot (offsets is the exclusive prefix, offsets[n] = total).
+// A worklist segment: up to kSegTiles consecutive tiles of one instance. The
+// finalize kernel compacts each segment into inst.seg[tile0 ...]; offsets is
+// the exclusive prefix of the segment counts.
+constexpr int kSegTiles = 1024;
+struct Seg {
+    int32_t inst;
+    int32_t tile0;
+    int32_t ntile;
+    int32_t pad;
+};
+
+// Per-segment partial results of the finalize kernel, reduced per instance
+// by the last workgroup in segment order (deterministic).
+struct SegPart {
+    unsigned long long c0, c1, cells;
+    double csum[kMaxFields];
+    double vmax;
+};
+
+// Per-instance segment range inside a group's segment table.
+struct InstSegs {
+    int32_t first, count;
+};
+
+// Item -> segment (offsets is the exclusive prefix, offsets[n] = total).
 __device__ __forceinline__ int find_slot(const int32_t* offsets, int n, int item) {
     int lo = 0, hi = n - 1;
     while (lo < hi) {

@@ -2,7 +2,7 @@
 //
 // Host orchestration of the gfx950 kernels: instances own their device state,
 // groups merge instances of one model into one worklist, every step is
-//   step kernel (fused stencil + detector + dilation atomics)
+//   step kernel (fused stencil + detector byte per tile)
 //   -> finalize kernel (parity flip, history, conserved shift / CFL dt,
 //      compaction of the next worklist),
 // on one HIP stream, optionally replayed from a captured hipGraph.
@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -81,9 +82,13 @@ struct Group {
     std::vector<int> ids;
     int model = 0, tile = 0, threads = 0;
     Inst* d_insts = nullptr;
+    Seg* d_segs = nullptr;
+    InstSegs* d_isegs = nullptr;
+    SegPart* d_parts = nullptr;
     int32_t* d_counts = nullptr;
     int32_t* d_offsets = nullptr;
     uint32_t* d_done = nullptr;
+    int nseg = 0;
     int64_t total_tiles = 0;
     int grid = 0;
 };
@@ -179,7 +184,7 @@ int64_t footprint_of(const dba_params& p) {
     int64_t b = 0;
     b += 2 * (int64_t)L.nf * L.n * 8;        // double-buffered fields
     b += (int64_t)L.naux * L.n * 8;          // stored neighbour state
-    b += 6 * (int64_t)L.nwords * 8;          // bitmaps
+    b += 4 * (int64_t)L.nwords * 8 + 2 * (int64_t)L.nt;  // bitmaps + detector bytes
     b += (int64_t)kMaxFields * L.nt * 8;     // tpart
     b += 2 * (int64_t)L.nt * 8;              // tspeed
     b += (int64_t)kMaxFields * 2 * L.nt * 8; // s_tile
@@ -213,6 +218,9 @@ void invalidate_groups(dba_ctx* ctx, int id) {
         if (std::find(it->first.begin(), it->first.end(), id) != it->first.end()) {
             Group* g = it->second;
             (void)hipFree(g->d_insts);
+            (void)hipFree(g->d_segs);
+            (void)hipFree(g->d_isegs);
+            (void)hipFree(g->d_parts);
             (void)hipFree(g->d_counts);
             (void)hipFree(g->d_offsets);
             (void)hipFree(g->d_done);
@@ -247,11 +255,20 @@ dba_status get_group(dba_ctx* ctx, const std::vector<int>& ids, Group** out) {
     Group* g = new Group();
     g->ids = ids;
     std::vector<Inst> hs;
-    for (int id : ids) {
-        Instance* in = ctx->inst[id];
+    std::vector<Seg> segs;
+    std::vector<InstSegs> isegs;
+    for (size_t k = 0; k < ids.size(); k++) {
+        Instance* in = ctx->inst[ids[k]];
         hs.push_back(in->h);
         g->total_tiles += in->h.nt;
+        InstSegs r{(int32_t)segs.size(), 0};
+        for (int t0 = 0; t0 < in->h.nt; t0 += kSegTiles) {
+            segs.push_back(Seg{(int32_t)k, t0, std::min(kSegTiles, in->h.nt - t0), 0});
+            r.count++;
+        }
+        isegs.push_back(r);
     }
+    g->nseg = (int)segs.size();
     Instance* first = ctx->inst[ids[0]];
     g->model = first->p.model;
     g->tile = first->p.tile_x;
@@ -261,11 +278,18 @@ dba_status get_group(dba_ctx* ctx, const std::vector<int>& ids, Group** out) {
     size_t n = ids.size();
     HIP_TRY(hipMalloc(&g->d_insts, sizeof(Inst) * n));
     HIP_TRY(hipMemcpy(g->d_insts, hs.data(), sizeof(Inst) * n, hipMemcpyHostToDevice));
-    HIP_TRY(hipMalloc(&g->d_counts, sizeof(int32_t) * n));
-    HIP_TRY(hipMalloc(&g->d_offsets, sizeof(int32_t) * (n + 1)));
+    size_t ns = segs.size();
+    HIP_TRY(hipMalloc(&g->d_segs, sizeof(Seg) * ns));
+    HIP_TRY(hipMemcpy(g->d_segs, segs.data(), sizeof(Seg) * ns, hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&g->d_isegs, sizeof(InstSegs) * n));
+    HIP_TRY(hipMemcpy(g->d_isegs, isegs.data(), sizeof(InstSegs) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&g->d_parts, sizeof(SegPart) * ns));
+    HIP_TRY(hipMemset(g->d_parts, 0, sizeof(SegPart) * ns));
+    HIP_TRY(hipMalloc(&g->d_counts, sizeof(int32_t) * ns));
+    HIP_TRY(hipMalloc(&g->d_offsets, sizeof(int32_t) * (ns + 1)));
     HIP_TRY(hipMalloc(&g->d_done, sizeof(uint32_t)));
     HIP_TRY(hipMemset(g->d_done, 0, sizeof(uint32_t)));
-    HIP_TRY(hipMemset(g->d_offsets, 0, sizeof(int32_t) * (n + 1)));
+    HIP_TRY(hipMemset(g->d_offsets, 0, sizeof(int32_t) * (ns + 1)));
     HIP_TRY(hipDeviceSynchronize());
     ctx->groups[ids] = g;
     *out = g;
@@ -277,13 +301,17 @@ dba_status get_group(dba_ctx* ctx, const std::vector<int>& ids, Group** out) {
 template <int TB, int NT>
 void launch_step_tb(const Group* g, hipStream_t s) {
     dim3 grid(g->grid), block(NT);
-    int n = (int)g->ids.size();
     switch (g->model) {
-        case DBA_MODEL_CH:
-            hipLaunchKernelGGL((ch_step_kernel<TB, NT>), grid, block, 0, s, g->d_insts, g->d_offsets, n);
+        case DBA_MODEL_CH: {
+            // the strip layout needs NT <= TB*TB
+            constexpr int NTC = NT <= TB * TB ? NT : TB * TB;
+            hipLaunchKernelGGL((ch_step_kernel<TB, NTC>), grid, dim3(NTC), 0, s, g->d_insts, g->d_segs,
+                               g->d_offsets, g->nseg);
             break;
+        }
         case DBA_MODEL_DENDRITE_K93:
-            hipLaunchKernelGGL((k93_step_kernel<TB, NT>), grid, block, 0, s, g->d_insts, g->d_offsets, n);
+            hipLaunchKernelGGL((k93_step_kernel<TB, NT>), grid, block, 0, s, g->d_insts, g->d_segs, g->d_offsets,
+                               g->nseg);
             break;
         default:
             break;
@@ -308,8 +336,9 @@ void launch_step(const Group* g, hipStream_t s) {
 }
 
 void launch_finalize(const Group* g, hipStream_t s, int rotate) {
-    hipLaunchKernelGGL((finalize_kernel<256>), dim3((unsigned)g->ids.size()), dim3(256), 0, s, g->d_insts,
-                       (int)g->ids.size(), g->d_counts, g->d_offsets, g->d_done, rotate);
+    hipLaunchKernelGGL(finalize_kernel, dim3((unsigned)g->nseg), dim3(kFinNT), 0, s, g->d_insts, g->d_segs,
+                       g->nseg, g->d_isegs, (int)g->ids.size(), g->d_parts, g->d_counts, g->d_offsets, g->d_done,
+                       rotate);
 }
 
 hipEvent_t pooled_event(dba_ctx* ctx) {
@@ -516,7 +545,7 @@ dba_status dba_instance_create(dba_ctx* ctx, const dba_params* pp, int32_t* id) 
     for (int a = 0; a < L.naux; a++) ALLOC(h.aux[a], (size_t)L.n);
     for (int s = 0; s < 2; s++) {
         ALLOC(h.a_cur[s], (size_t)L.nwords);
-        ALLOC(h.a_next[s], (size_t)L.nwords);
+        ALLOC(h.nflag[s], (size_t)L.nt);
         ALLOC(h.par[s], (size_t)L.nwords);
     }
     ALLOC(h.tpart, (size_t)kMaxFields * L.nt);

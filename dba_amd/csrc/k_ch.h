@@ -6,7 +6,9 @@
 // (example_Reactive_ModelB/kernel.cu:519-654, 469-517, 657-713), plus the
 // conserved change capture (example_Reactive_ModelB/Conserved/kernel.cu:630-702).
 //
-// One workgroup per active tile per worklist item:
+// One workgroup per awake tile (worklist item):
+//   0. resolve the 3x3 neighbour tiles once (owner tile, buffer parity, awake,
+//      conserved shift) into LDS;
 //   1. stage c on the tile plus a 2-cell halo in LDS (coalesced rows; halo cells
 //      come from each owner tile's current buffer, minus its conserved shift);
 //   2. mu on the tile plus a 1-cell ring: computed here for the tile and for ring
@@ -14,131 +16,170 @@
 //      cells of sleeping neighbours use the mu they stored when they last ran,
 //      which is what the reference's full mu array holds for them;
 //   3. c' = c + dt (M lap mu - Gamma (c - cbar)) into the other buffer;
-//   4. detector any(|lap c| > eps) -> atomicOr dilation into the next bitmap;
-//      the tile's own edge-ring mu is stored for its neighbours.
+//   4. detector any(|lap c| > eps) -> one byte per tile; the tile's own
+//      edge-ring mu is stored for its neighbours.
 // HBM traffic per cell: read c, write c' (16 B) + the mu ring (4(T-1)/T^2 of cells).
 #pragma once
 #include "common.h"
 
 namespace dba {
 
-template <int W>
-__device__ __forceinline__ double lds_lap5(const double* s, int x, int y, double invdx2) {
-    double s1 = (s[(x + 1) + y * W] + s[(x - 1) + y * W]) + (s[x + (y + 1) * W] + s[x + (y - 1) * W]);
-    return (s1 - 4.0 * s[x + y * W]) * invdx2;
+// Column-strip register blocking: thread (tx, ty) owns rows ty*R .. ty*R+R-1
+// of column tx, R = TB*TB/NT. A 9-point stencil over its R cells reads
+// 3 (R + 2) LDS values instead of 9 R.
+template <int W, int R, bool NINE>
+__device__ __forceinline__ void strip_lap(const double* s, int cx, int cy0, double inv, double* out) {
+    double L[R + 2], C[R + 2], Rt[R + 2];
+#pragma unroll
+    for (int r = 0; r < R + 2; r++) {
+        const int row = (cy0 - 1 + r) * W;
+        L[r] = s[row + cx - 1];
+        C[r] = s[row + cx];
+        Rt[r] = s[row + cx + 1];
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const double s1 = (Rt[r + 1] + L[r + 1]) + (C[r + 2] + C[r]);
+        if (NINE) {
+            const double s2 = (Rt[r + 2] + L[r + 2]) + (Rt[r] + L[r]);
+            out[r] = ((4.0 * s1 + s2) - 20.0 * C[r + 1]) * inv;
+        } else {
+            out[r] = (s1 - 4.0 * C[r + 1]) * inv;
+        }
+    }
 }
 
-template <int W>
-__device__ __forceinline__ double lds_lap9(const double* s, int x, int y, double inv6dx2) {
-    double s1 = (s[(x + 1) + y * W] + s[(x - 1) + y * W]) + (s[x + (y + 1) * W] + s[x + (y - 1) * W]);
-    double s2 = (s[(x + 1) + (y + 1) * W] + s[(x - 1) + (y + 1) * W]) +
-                (s[(x + 1) + (y - 1) * W] + s[(x - 1) + (y - 1) * W]);
-    return ((4.0 * s1 + s2) - 20.0 * s[x + y * W]) * inv6dx2;
+template <int TB, int NT, bool NINE>
+__device__ __forceinline__ void ch_tile(const Inst& I, int t, NbTable& nb, double* sc, double* smu, double* sred,
+                                        int* sflag) {
+    constexpr int W2 = TB + 4;
+    constexpr int W1 = TB + 2;
+    constexpr int R = TB * TB / NT;
+    constexpr int NH2 = W2 * W2 - TB * TB;  // halo cells of the c region
+    constexpr int NH1 = W1 * W1 - TB * TB;  // ring cells of the mu region
+    const int tid = threadIdx.x;
+    const int tx = tid % TB, ty = tid / TB;
+    const int nx = I.nx, ny = I.ny, ntx = I.ntx, bcx = I.bc_x, bcy = I.bc_y;
+    const int x0 = (t % ntx) * TB, y0 = (t / ntx) * TB;
+    const bool regular = I.mode == 0;
+    const bool cons = I.mode == 2;
+    const double* __restrict__ b0 = I.buf[0][0];
+    const double* __restrict__ b1 = I.buf[1][0];
+    double* __restrict__ mu_store = I.aux[0];
+    const double kappa = I.kappa, eps = I.thresh_a;
+    const double inv = NINE ? I.inv6dx2 : I.invdx2;
+    nb_fill<TB>(nb, I, t, x0, y0, regular, 1, cons, 0);
+    __syncthreads();
+    const int pself = nb.par[0][4];
+    const double sh_self = nb.shift[4];
+    const double* __restrict__ cur = pself ? b1 : b0;
+    double* __restrict__ nxt = pself ? I.buf[0][0] : I.buf[1][0];
+
+    // 1. c: the thread's strip from HBM into registers and LDS, then the halo
+    double c[R];
+    const int gx = x0 + tx;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        c[r] = cur[gx + (int64_t)(y0 + ty * R + r) * nx] - sh_self;
+        sc[(tx + 2) + (ty * R + r + 2) * W2] = c[r];
+    }
+    for (int k = tid; k < NH2; k += NT) {
+        int lx, ly;
+        if (k < 2 * W2) { ly = k / W2; lx = k % W2; }                      // two rows below
+        else if (k < 4 * W2) { ly = TB + 2 + (k - 2 * W2) / W2; lx = (k - 2 * W2) % W2; }  // two rows above
+        else { const int q = k - 4 * W2; ly = 2 + q / 4; const int c4 = q % 4; lx = c4 < 2 ? c4 : TB + c4; }
+        const int d = nb_dir1<TB, 2>(lx) + 3 * nb_dir1<TB, 2>(ly);
+        const int mx = map_index(x0 + lx - 2, nx, bcx), my = map_index(y0 + ly - 2, ny, bcy);
+        sc[lx + ly * W2] = (nb.par[0][d] ? b1 : b0)[mx + (int64_t)my * nx] - nb.shift[d];
+    }
+    __syncthreads();
+
+    // 2. mu on the strip (+ detector), then on the 1-cell ring
+    int flag = 0;
+    {
+        double lc[R];
+        strip_lap<W2, R, NINE>(sc, tx + 2, ty * R + 2, inv, lc);
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const double cv = c[r];
+            const double mu = (cv * cv * cv - cv) - kappa * lc[r];
+            flag |= fabs(lc[r]) > eps;
+            const int ly = ty * R + r;
+            smu[(tx + 1) + (ly + 1) * W1] = mu;
+            if (!regular && (tx == 0 || tx == TB - 1 || ly == 0 || ly == TB - 1))
+                mu_store[(x0 + tx) + (int64_t)(y0 + ly) * nx] = mu;
+        }
+    }
+    for (int k = tid; k < NH1; k += NT) {
+        int lx, ly;
+        if (k < W1) { ly = 0; lx = k; }
+        else if (k < 2 * W1) { ly = TB + 1; lx = k - W1; }
+        else { const int q = k - 2 * W1; ly = 1 + q / 2; lx = (q % 2) ? TB + 1 : 0; }
+        const int d = nb_dir1<TB, 1>(lx) + 3 * nb_dir1<TB, 1>(ly);
+        double mu;
+        if (nb.act[0][d]) {
+            const double lcv = NINE ? lds_lap9<W2>(sc, lx + 1, ly + 1, inv) : lds_lap5<W2>(sc, lx + 1, ly + 1, inv);
+            const double cv = sc[(lx + 1) + (ly + 1) * W2];
+            mu = (cv * cv * cv - cv) - kappa * lcv;
+        } else {
+            const int mx = map_index(x0 + lx - 1, nx, bcx), my = map_index(y0 + ly - 1, ny, bcy);
+            mu = mu_store[mx + (int64_t)my * nx];
+        }
+        smu[lx + ly * W1] = mu;
+    }
+    __syncthreads();
+
+    // 3. c' on the strip
+    double part = 0.0;
+    {
+        double lm[R];
+        strip_lap<W1, R, NINE>(smu, tx + 1, ty * R + 1, inv, lm);
+        const double dt = I.dt, mob = I.mobility, rg = I.react_gamma, cbar = I.cbar;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const double change = dt * (mob * lm[r] - rg * (c[r] - cbar));
+            nxt[gx + (int64_t)(y0 + ty * R + r) * nx] = c[r] + change;
+            part += change;
+        }
+    }
+
+    // 4. detector byte, conserved bookkeeping
+    if (!regular) {
+        const int any = block_any<NT>(flag, sflag);
+        if (tid == 0) I.nflag[0][t] = (uint8_t)any;
+    }
+    if (cons) {
+        const double tot = block_sum<NT>(part, sred);
+        if (tid == 0) {
+            I.tpart[t] = tot;
+            I.s_tile[(pself ^ 1) * I.nt + t] = I.scal[8];
+        }
+    }
+    __syncthreads();
 }
 
 template <int TB, int NT>
 __global__ __launch_bounds__(NT) void ch_step_kernel(const Inst* __restrict__ insts,
-                                                    const int32_t* __restrict__ offsets, int n_inst) {
+                                                    const Seg* __restrict__ segs,
+                                                    const int32_t* __restrict__ offsets, int nseg) {
+    static_assert(NT % TB == 0 && TB * TB % NT == 0, "thread layout needs TB | NT | TB*TB");
     constexpr int W2 = TB + 4;
     constexpr int W1 = TB + 2;
     __shared__ double sc[W2 * W2];
     __shared__ double smu[W1 * W1];
     __shared__ double sred[NT / 64 + 1];
     __shared__ int sflag[NT / 64 + 1];
+    __shared__ NbTable nb;
 
-    const int total = offsets[n_inst];
+    const int total = offsets[nseg];
     const int G = gridDim.x;
-    const int tid = threadIdx.x;
     for (int item = xcd_remap(blockIdx.x, G); item < total; item += G) {
-        const int slot = find_slot(offsets, n_inst, item);
-        const Inst& I = insts[slot];
-        const int t = I.seg[item - offsets[slot]];
-        const int nx = I.nx, ny = I.ny;
-        const int tix = t % I.ntx, tiy = t / I.ntx;
-        const int x0 = tix * TB, y0 = tiy * TB;
-        const bool regular = I.mode == 0;
-        const bool cons = I.mode == 2;
-        const int pself = bit_of(I.par[0], t);
-        const double* cur = I.buf[pself][0];
-        double* nxt = I.buf[pself ^ 1][0];
-        const double S = cons ? I.scal[8] : 0.0;
-        const double shift_self = cons ? (S - I.s_tile[pself * I.nt + t]) : 0.0;
-        const bool nine = I.stencil != 5;
-
-        // 1. c on the tile + 2-cell halo
-        for (int k = tid; k < W2 * W2; k += NT) {
-            const int lx = k % W2, ly = k / W2;
-            const int gx = x0 + lx - 2, gy = y0 + ly - 2;
-            double v;
-            if (lx >= 2 && lx < TB + 2 && ly >= 2 && ly < TB + 2) {
-                v = cur[gx + (int64_t)gy * nx] - shift_self;
-            } else {
-                const int mx = map_index(gx, nx, I.bc_x), my = map_index(gy, ny, I.bc_y);
-                const int tt = mx / TB + (my / TB) * I.ntx;
-                const int pp = bit_of(I.par[0], tt);
-                v = I.buf[pp][0][mx + (int64_t)my * nx];
-                if (cons) v -= (S - I.s_tile[pp * I.nt + tt]);
-            }
-            sc[k] = v;
-        }
-        __syncthreads();
-
-        // 2. mu on the tile + 1-cell ring; detector on the tile
-        int flag = 0;
-        for (int k = tid; k < W1 * W1; k += NT) {
-            const int lx = k % W1, ly = k / W1;
-            const bool self = lx >= 1 && lx <= TB && ly >= 1 && ly <= TB;
-            bool fresh = self || regular;
-            int64_t gidx = 0;
-            if (!self) {
-                const int mx = map_index(x0 + lx - 1, nx, I.bc_x), my = map_index(y0 + ly - 1, ny, I.bc_y);
-                const int tt = mx / TB + (my / TB) * I.ntx;
-                gidx = mx + (int64_t)my * nx;
-                if (!fresh) fresh = (tt == t) || bit_of(I.a_cur[0], tt);
-            }
-            double mu;
-            if (fresh) {
-                const double lc = nine ? lds_lap9<W2>(sc, lx + 1, ly + 1, I.inv6dx2)
-                                       : lds_lap5<W2>(sc, lx + 1, ly + 1, I.invdx2);
-                const double cv = sc[(lx + 1) + (ly + 1) * W2];
-                mu = (cv * cv * cv - cv) - I.kappa * lc;
-                if (self) {
-                    flag |= fabs(lc) > I.thresh_a;
-                    if (!regular && (lx == 1 || lx == TB || ly == 1 || ly == TB))
-                        I.aux[0][(x0 + lx - 1) + (int64_t)(y0 + ly - 1) * nx] = mu;
-                }
-            } else {
-                mu = I.aux[0][gidx];
-            }
-            smu[k] = mu;
-        }
-        __syncthreads();
-
-        // 3. c' on the tile
-        double part = 0.0;
-        for (int k = tid; k < TB * TB; k += NT) {
-            const int lx = k % TB, ly = k / TB;
-            const double lm = nine ? lds_lap9<W1>(smu, lx + 1, ly + 1, I.inv6dx2)
-                                   : lds_lap5<W1>(smu, lx + 1, ly + 1, I.invdx2);
-            const double cv = sc[(lx + 2) + (ly + 2) * W2];
-            const double change = I.dt * (I.mobility * lm - I.react_gamma * (cv - I.cbar));
-            nxt[(x0 + lx) + (int64_t)(y0 + ly) * nx] = cv + change;
-            part += change;
-        }
-
-        // 4. detector + dilation, conserved bookkeeping
-        if (!regular) {
-            const int any = block_any<NT>(flag, sflag);
-            if (tid == 0 && any) dilate_into(I.a_next[0], I, t, 0);
-        }
-        if (cons) {
-            const double tot = block_sum<NT>(part, sred);
-            if (tid == 0) {
-                I.tpart[t] = tot;
-                I.s_tile[(pself ^ 1) * I.nt + t] = S;
-            }
-        }
-        __syncthreads();
+        const int s = nseg == 1 ? 0 : find_slot(offsets, nseg, item);
+        const Seg sg = segs[s];
+        const Inst& I = insts[sg.inst];
+        const int t = I.seg[sg.tile0 + (item - offsets[s])];
+        if (I.stencil != 5) ch_tile<TB, NT, true>(I, t, nb, sc, smu, sred, sflag);
+        else ch_tile<TB, NT, false>(I, t, nb, sc, smu, sred, sflag);
     }
 }
 

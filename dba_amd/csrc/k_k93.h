@@ -43,7 +43,8 @@ __device__ __forceinline__ void k93_aniso(const Inst& I, double gx, double gy, d
 
 template <int TB, int NT>
 __global__ __launch_bounds__(NT) void k93_step_kernel(const Inst* __restrict__ insts,
-                                                     const int32_t* __restrict__ offsets, int n_inst) {
+                                                     const Seg* __restrict__ segs,
+                                                     const int32_t* __restrict__ offsets, int nseg) {
     constexpr int W2 = TB + 4;
     constexpr int W1 = TB + 2;
     __shared__ double sp[W2 * W2];
@@ -52,63 +53,56 @@ __global__ __launch_bounds__(NT) void k93_step_kernel(const Inst* __restrict__ i
     __shared__ double sFy[W1 * W1];
     __shared__ double sE2[TB * TB];
     __shared__ int sflag[NT / 64 + 1];
+    __shared__ NbTable nb;
 
-    const int total = offsets[n_inst];
+    const int total = offsets[nseg];
     const int G = gridDim.x;
     const int tid = threadIdx.x;
     for (int item = xcd_remap(blockIdx.x, G); item < total; item += G) {
-        const int slot = find_slot(offsets, n_inst, item);
-        const Inst& I = insts[slot];
-        const int t = I.seg[item - offsets[slot]];
-        const int nx = I.nx, ny = I.ny;
-        const int x0 = (t % I.ntx) * TB, y0 = (t / I.ntx) * TB;
+        const int s = nseg == 1 ? 0 : find_slot(offsets, nseg, item);
+        const Seg sg = segs[s];
+        const Inst& I = insts[sg.inst];
+        const int t = I.seg[sg.tile0 + (item - offsets[s])];
+        const int nx = I.nx, ny = I.ny, ntx = I.ntx, bcx = I.bc_x, bcy = I.bc_y;
+        const int x0 = (t % ntx) * TB, y0 = (t / ntx) * TB;
         const bool regular = I.mode == 0;
-        const int ap = regular ? 1 : bit_of(I.a_cur[0], t);
-        const int at = regular ? 1 : bit_of(I.a_cur[1], t);
-        const int pp0 = bit_of(I.par[0], t);
-        const int pp1 = bit_of(I.par[1], t);
+        const double* __restrict__ p0 = I.buf[0][0];
+        const double* __restrict__ p1 = I.buf[1][0];
+        const double* __restrict__ T0 = I.buf[0][1];
+        const double* __restrict__ T1 = I.buf[1][1];
+        nb_fill<TB>(nb, I, t, x0, y0, regular, 2, false, 0);
+        __syncthreads();
+        const int ap = nb.act[0][4], at = nb.act[1][4];
+        const int pp0 = nb.par[0][4], pp1 = nb.par[1][4];
+        double* __restrict__ pnew = pp0 ? I.buf[0][0] : I.buf[1][0];
+        double* __restrict__ Tnew = pp1 ? I.buf[0][1] : I.buf[1][1];
 
-        // stage phi (2-cell halo) when p-awake; T (1-cell halo) when T-awake,
-        // and phi / T of the tile itself always (the T update needs phi - phi')
+        // stage phi (2-cell halo) when p-awake, T (1-cell halo) when T-awake,
+        // and phi / T of the tile itself always (the T update needs phi' - phi)
         for (int k = tid; k < W2 * W2; k += NT) {
             const int lx = k % W2, ly = k / W2;
-            const bool self = lx >= 2 && lx < TB + 2 && ly >= 2 && ly < TB + 2;
-            if (!self && !ap) continue;
-            const int gx = x0 + lx - 2, gy = y0 + ly - 2;
-            double v;
-            if (self) {
-                v = I.buf[pp0][0][gx + (int64_t)gy * nx];
-            } else {
-                const int mx = map_index(gx, nx, I.bc_x), my = map_index(gy, ny, I.bc_y);
-                const int tt = mx / TB + (my / TB) * I.ntx;
-                v = I.buf[bit_of(I.par[0], tt)][0][mx + (int64_t)my * nx];
-            }
-            sp[k] = v;
+            const int d = nb_dir1<TB, 2>(lx) + 3 * nb_dir1<TB, 2>(ly);
+            if (d != 4 && !ap) continue;
+            const int mx = map_index(x0 + lx - 2, nx, bcx), my = map_index(y0 + ly - 2, ny, bcy);
+            sp[k] = (nb.par[0][d] ? p1 : p0)[mx + (int64_t)my * nx];
         }
         for (int k = tid; k < W1 * W1; k += NT) {
             const int lx = k % W1, ly = k / W1;
-            const bool self = lx >= 1 && lx <= TB && ly >= 1 && ly <= TB;
-            if (!self && !at) continue;
-            const int gx = x0 + lx - 1, gy = y0 + ly - 1;
-            double v;
-            if (self) {
-                v = I.buf[pp1][1][gx + (int64_t)gy * nx];
-            } else {
-                const int mx = map_index(gx, nx, I.bc_x), my = map_index(gy, ny, I.bc_y);
-                const int tt = mx / TB + (my / TB) * I.ntx;
-                v = I.buf[bit_of(I.par[1], tt)][1][mx + (int64_t)my * nx];
-            }
-            sT[k] = v;
+            const int d = nb_dir1<TB, 1>(lx) + 3 * nb_dir1<TB, 1>(ly);
+            if (d != 4 && !at) continue;
+            const int mx = map_index(x0 + lx - 1, nx, bcx), my = map_index(y0 + ly - 1, ny, bcy);
+            sT[k] = (nb.par[1][d] ? T1 : T0)[mx + (int64_t)my * nx];
         }
         __syncthreads();
 
         // eps eps' grad phi on the tile + ring, eps^2 on the tile
         if (ap) {
+            const double inv2dx = I.inv2dx;
             for (int k = tid; k < W1 * W1; k += NT) {
                 const int lx = k % W1, ly = k / W1;
                 const int cx = lx + 1, cy = ly + 1;
-                const double gx = (sp[(cx + 1) + cy * W2] - sp[(cx - 1) + cy * W2]) * I.inv2dx;
-                const double gy = (sp[cx + (cy + 1) * W2] - sp[cx + (cy - 1) * W2]) * I.inv2dx;
+                const double gx = (sp[(cx + 1) + cy * W2] - sp[(cx - 1) + cy * W2]) * inv2dx;
+                const double gy = (sp[cx + (cy + 1) * W2] - sp[cx + (cy - 1) * W2]) * inv2dx;
                 double e, a;
                 k93_aniso(I, gx, gy, &e, &a);
                 sFx[k] = a * gx;
@@ -119,6 +113,9 @@ __global__ __launch_bounds__(NT) void k93_step_kernel(const Inst* __restrict__ i
         __syncthreads();
 
         int fp = 0, fT = 0;
+        const double inv2dx = I.inv2dx, invdx2 = I.invdx2, alpha_pi = I.alpha_pi, gam = I.gamma;
+        const double t_eq = I.t_eq, dt_tau = I.dt_tau, dt = I.dt, lk = I.latent_k;
+        const double mp = I.thresh_a, mT = I.thresh_b;
         for (int k = tid; k < TB * TB; k += NT) {
             const int lx = k % TB, ly = k / TB;
             const int64_t gi = (x0 + lx) + (int64_t)(y0 + ly) * nx;
@@ -128,20 +125,20 @@ __global__ __launch_bounds__(NT) void k93_step_kernel(const Inst* __restrict__ i
             if (ap) {
                 const int cx = lx + 2, cy = ly + 2, fx = lx + 1, fy = ly + 1;
                 const double cross = ((sFx[fx + (fy + 1) * W1] - sFx[fx + (fy - 1) * W1]) -
-                                      (sFy[(fx + 1) + fy * W1] - sFy[(fx - 1) + fy * W1])) * I.inv2dx;
-                const double lap = lds_lap5<W2>(sp, cx, cy, I.invdx2);
-                const double m = I.alpha_pi * atan(I.gamma * (I.t_eq - Tv));
+                                      (sFy[(fx + 1) + fy * W1] - sFy[(fx - 1) + fy * W1])) * inv2dx;
+                const double lap = lds_lap5<W2>(sp, cx, cy, invdx2);
+                const double m = alpha_pi * atan(gam * (t_eq - Tv));
                 const double rhs = (cross + sE2[k] * lap) + (ph * (1.0 - ph)) * ((ph - 0.5) + m);
-                pn = ph + I.dt_tau * rhs;
-                I.buf[pp0 ^ 1][0][gi] = pn;
-                fp |= (pn >= I.thresh_a && pn <= (1.0 - I.thresh_a));
+                pn = ph + dt_tau * rhs;
+                pnew[gi] = pn;
+                fp |= (pn >= mp && pn <= (1.0 - mp));
             }
             if (at) {
-                const double lapT = lds_lap5<W1>(sT, lx + 1, ly + 1, I.invdx2);
-                const double t1 = Tv + I.dt * lapT;
-                const double Tn = ap ? t1 + I.latent_k * (pn - ph) : t1;
-                I.buf[pp1 ^ 1][1][gi] = Tn;
-                fT |= fabs(Tn - Tv) >= I.thresh_b;
+                const double lapT = lds_lap5<W1>(sT, lx + 1, ly + 1, invdx2);
+                const double t1 = Tv + dt * lapT;
+                const double Tn = ap ? t1 + lk * (pn - ph) : t1;
+                Tnew[gi] = Tn;
+                fT |= fabs(Tn - Tv) >= mT;
             }
         }
 
@@ -149,8 +146,8 @@ __global__ __launch_bounds__(NT) void k93_step_kernel(const Inst* __restrict__ i
             const int anyp = block_any<NT>(fp, sflag);
             const int anyT = block_any<NT>(fT, sflag);
             if (tid == 0) {
-                if (ap && anyp) dilate_into(I.a_next[0], I, t, 0);
-                if (at && anyT) dilate_into(I.a_next[1], I, t, 0);
+                if (ap) I.nflag[0][t] = (uint8_t)anyp;
+                if (at) I.nflag[1][t] = (uint8_t)anyT;
             }
         }
         __syncthreads();
