@@ -7,6 +7,8 @@ Workloads (--workload):
             Gamma=1e-3, cbar=0, c = U[-0.05, 0.05] (seed 1234).
   ch_cons   same, conserved DBA.
   dendrite  2D Kobayashi dendrite 512^2 FP64, 16x16 tiles, DBA.
+  kh        compressible Euler Kelvin-Helmholtz 2048^2 FP64, gamma 1.4, CFL 0.4,
+            reflective y walls, 16x16 tiles, density-gradient wake-up (eps 0.1).
 A step is one DBA time step of the whole instance (stencil kernel + finalize
 kernel), replayed from a captured hipGraph. Inputs are resident in HBM before
 the timed region. value = active cell-updates/s over the timed steps (device
@@ -14,7 +16,7 @@ counter of cells in awake tiles).
 
 roofline: the step kernel's algorithmic bytes per launch (B_alg x active cells
 of that launch: CH fused pass 16 B = read c + write c'; dendrite 32 B = read and
-write phi and T) over its average duration, timed with hipEvents on the
+write phi and T; KH 64 B = read and write the 4 conserved fields) over its average duration, timed with hipEvents on the
 library stream in a second, eager pass of the same workload.
 cpu_baseline: the oracle's regular-mode loop (serial and y-slab threads on the
 host cores) on a bounded sample of the same grid.
@@ -34,6 +36,7 @@ WORKLOADS = {
     "ch": dict(model="ch", n=1024, tile=32, mode="dba", b_alg=16, warm=500, steps=5000),
     "ch_cons": dict(model="ch", n=1024, tile=32, mode="conserved", b_alg=16, warm=500, steps=5000),
     "dendrite": dict(model="k93", n=512, tile=16, mode="dba", b_alg=32, warm=200, steps=4000),
+    "kh": dict(model="kh", n=2048, tile=16, mode="dba", b_alg=64, warm=100, steps=1000),
 }
 
 
@@ -44,6 +47,9 @@ def make(workload):
     if w["model"] == "ch":
         p = configs.ch_params(n=w["n"], tile=w["tile"], mode=mode)
         fields = [configs.ch_ic(w["n"], w["n"])]
+    elif w["model"] == "kh":
+        p = configs.kh_params(n=w["n"], tile=w["tile"], mode=mode)
+        fields = configs.kh_ic(w["n"])
     else:
         p = configs.dendrite_k93_params(n=w["n"], tile=w["tile"], mode=mode)
         fields = list(configs.dendrite_k93_ic(w["n"], w["n"]))

@@ -101,3 +101,98 @@ def ch_params(n=1024, tile=32, mode=abi.MODE_DBA, stencil=9, react_gamma=1e-3, c
 
 def ch_ic(nx, ny, cbar=0.0, amp=0.05, seed=1234):
     return cbar + amp * (2.0 * uniform01(seed, nx * ny) - 1.0).reshape(ny, nx)
+
+
+def kh_params(n=2048, tile=16, mode=abi.MODE_DBA, gas_gamma=1.4, cfl=0.4, walls=True, metric=abi.KH_METRIC_DENSITY,
+              eps=0.1, dt=2e-4, dt_mode=abi.DT_CFL, threads=0, history_cap=0, ny=None):
+    """Compressible Euler on the unit square (dx = dy = 1/n)."""
+    p = base_params(abi.MODEL_KH, n, ny or n, 1, tile, mode)
+    p.bc_x = abi.BC_PERIODIC
+    p.bc_y = abi.BC_WALL if walls else abi.BC_PERIODIC
+    p.dx = p.dy = p.dz = 1.0 / n
+    p.dt = dt
+    p.dt_mode = dt_mode
+    p.gas_gamma = gas_gamma
+    p.cfl = cfl
+    p.metric_kind = metric
+    p.thresh_a = eps
+    p.threads = threads
+    p.history_cap = history_cap
+    return p
+
+
+def kh_reference_params(n=512, tile=8, mode=abi.MODE_DBA, dt=2e-4, dx=0.002, eps=0.0, history_cap=0):
+    """The reference KH model (example_Kelvin_Helmholtz/Input.txt): gamma = 5/3,
+    periodic in x and y, fixed dt, 8x8 blocks, sum-of-8-gradients detector."""
+    p = base_params(abi.MODEL_KH, n, n, 1, tile, mode)
+    p.bc_x = p.bc_y = abi.BC_PERIODIC
+    p.dx = p.dy = p.dz = dx
+    p.dt = dt
+    p.dt_mode = abi.DT_FIXED
+    p.gas_gamma = 5.0 / 3.0
+    p.metric_kind = abi.KH_METRIC_ALLGRAD
+    p.thresh_a = eps
+    p.history_cap = history_cap
+    return p
+
+
+def kh_conserved(rho, vx, vy, pr, gas_gamma, vol):
+    """getConserved (example_Kelvin_Helmholtz/KH_DBA.cu:363-367), same operation order."""
+    mass = rho * vol
+    momx = rho * vx * vol
+    momy = rho * vy * vol
+    energy = (pr / (gas_gamma - 1.0) + 0.5 * rho * (vx * vx + vy * vy)) * vol
+    return [mass, momx, momy, energy]
+
+
+def kh_primitive(U, gas_gamma, vol):
+    """getPrimitive (example_Kelvin_Helmholtz/KH_DBA.cu:401-405)."""
+    mass, momx, momy, energy = U
+    rho = mass / vol
+    vx = momx / rho / vol
+    vy = momy / rho / vol
+    pr = (energy / vol - 0.5 * rho * (vx * vx + vy * vy)) * (gas_gamma - 1.0)
+    return rho, vx, vy, pr
+
+
+def kh_ic(n, gas_gamma=1.4, ny=None):
+    """Sharp shear layers: rho=2, vx=+0.5 for |y-1/2|<1/4, else rho=1, vx=-0.5; p=2.5;
+    vy = 0.01 sin(4 pi x) (exp(-(y-1/4)^2/0.005) + exp(-(y-3/4)^2/0.005)), cell centres
+    (the sharp form of example_Kelvin_Helmholtz/CPU_version/KH.py:222-225, the two
+    Gaussians summed). Returns conserved fields."""
+    ny = ny or n
+    x = (np.arange(n) + 0.5) / n
+    y = (np.arange(ny) + 0.5) / ny
+    X, Y = np.meshgrid(x, y)
+    inner = np.abs(Y - 0.5) < 0.25
+    rho = 1.0 + inner
+    vx = -0.5 + inner
+    vy = 0.01 * np.sin(4 * np.pi * X) * (np.exp(-(Y - 0.25) ** 2 / 0.005) + np.exp(-(Y - 0.75) ** 2 / 0.005))
+    pr = 2.5 * np.ones_like(X)
+    return kh_conserved(rho, vx, vy, pr, gas_gamma, (1.0 / n) * (1.0 / n))
+
+
+def kh_reference_ic(n, dx=0.002):
+    """Smooth profile of example_Kelvin_Helmholtz/KH_DBA.cu:1102-1125 (integer y / Ny)."""
+    y = np.arange(n, dtype=np.float64)[:, None] * np.ones((1, n))
+    x = np.ones((n, 1)) * np.arange(n, dtype=np.float64)[None, :]
+    rho1, rho2, vx1, vx2 = 1.0, 2.0, 0.5, -0.5
+    rho_m, vx_m = 0.5 * (rho1 - rho2), 0.5 * (vx1 - vx2)
+    ys = 1.0 * y / n
+    rho = np.empty_like(y)
+    vx = np.empty_like(y)
+    a = y < n // 4
+    b = (y >= n // 4) & (y < n // 2)
+    c = (y >= n // 2) & (y < 3 * n // 4)
+    d = y >= 3 * n // 4
+    rho[a] = rho1 - rho_m * np.exp((ys[a] - 0.25) / 0.025)
+    vx[a] = vx1 - vx_m * np.exp((ys[a] - 0.25) / 0.025)
+    rho[b] = rho2 + rho_m * np.exp((-ys[b] + 0.25) / 0.025)
+    vx[b] = vx2 + vx_m * np.exp((-ys[b] + 0.25) / 0.025)
+    rho[c] = rho2 + rho_m * np.exp((ys[c] - 0.75) / 0.025)
+    vx[c] = vx2 + vx_m * np.exp((ys[c] - 0.75) / 0.025)
+    rho[d] = rho1 - rho_m * np.exp((-ys[d] + 0.75) / 0.025)
+    vx[d] = vx1 - vx_m * np.exp((-ys[d] + 0.75) / 0.025)
+    vy = 0.01 * np.sin(4.0 * np.pi * x / n)
+    pr = 2.5 * np.ones_like(y)
+    return kh_conserved(rho, vx, vy, pr, 5.0 / 3.0, dx * dx)

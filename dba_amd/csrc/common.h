@@ -126,26 +126,27 @@ struct NbTable {
     int tile[9];
     int par[2][9];
     int act[2][9];
-    double shift[9];
+    double shift[kMaxFields][9];
 };
 
 // Fill by threads 0..8; caller synchronises.
 template <int TB>
 __device__ __forceinline__ void nb_fill(NbTable& nb, const Inst& I, int t, int x0, int y0, bool regular,
-                                        int nsets, bool cons, int consf) {
+                                        int nsets, int ncons) {
     const int d = threadIdx.x;
     if (d >= 9) return;
     const int dx = d % 3 - 1, dy = d / 3 - 1;
     const int rx = dx < 0 ? x0 - 1 : (dx > 0 ? x0 + TB : x0);
     const int ry = dy < 0 ? y0 - 1 : (dy > 0 ? y0 + TB : y0);
-    const int mx = map_index(rx, I.nx, I.bc_x), my = map_index(ry, I.ny, I.bc_y);
+    const int mx = map_index(rx, I.nx, I.bc_x), my = map_index(ry, I.ny, I.bc_y == 0 ? 0 : 1);
     const int tt = mx / TB + (my / TB) * I.ntx;
     nb.tile[d] = tt;
     for (int s = 0; s < nsets; s++) {
         nb.par[s][d] = bit_of(I.par[s], tt);
         nb.act[s][d] = regular ? 1 : bit_of(I.a_cur[s], tt);
     }
-    nb.shift[d] = cons ? (I.scal[8 + consf] - I.s_tile[((int64_t)consf * 2 + nb.par[0][d]) * I.nt + tt]) : 0.0;
+    for (int f = 0; f < kMaxFields; f++)
+        nb.shift[f][d] = f < ncons ? (I.scal[8 + f] - I.s_tile[((int64_t)f * 2 + nb.par[0][d]) * I.nt + tt]) : 0.0;
 }
 
 // Direction index of a cell at local coordinate l (0..W-1) of a region whose

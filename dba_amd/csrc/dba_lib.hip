@@ -23,6 +23,7 @@
 #include "k_finalize.h"
 #include "k_ch.h"
 #include "k_k93.h"
+#include "k_kh.h"
 
 using namespace dba;
 
@@ -58,6 +59,7 @@ int nfields_of(int model) {
     switch (model) {
         case DBA_MODEL_CH: return 1;
         case DBA_MODEL_DENDRITE_K93: return 2;
+        case DBA_MODEL_KH: return 4;
         default: return 0;
     }
 }
@@ -75,6 +77,7 @@ struct Instance {
     std::vector<void*> allocs;   // device allocations owned by this instance
     int64_t footprint = 0;
     int64_t ncell = 0;
+    bool needs_prime = false;    // KH: per-tile wave speeds must be recomputed before the next step
     Inst* d_self = nullptr;      // single-instance descriptor (downloads, checks)
 };
 
@@ -144,6 +147,20 @@ dba_status validate(const dba_params& p) {
             return fail(DBA_ERR_INVALID, "grid %dx%d must be a multiple of the tile %d", p.nx, p.ny, p.tile_x);
         if (p.nx < p.tile_x * 1 || p.ny < p.tile_y) return fail(DBA_ERR_INVALID, "grid smaller than a tile");
     }
+    if (p.model == DBA_MODEL_KH) {
+        if (p.nz != 1) return fail(DBA_ERR_INVALID, "KH is 2D (nz must be 1)");
+        if (p.tile_x != p.tile_y || (p.tile_x != 8 && p.tile_x != 16))
+            return fail(DBA_ERR_INVALID, "KH tile must be 8x8 or 16x16");
+        if (p.nx % p.tile_x || p.ny % p.tile_y)
+            return fail(DBA_ERR_INVALID, "grid %dx%d must be a multiple of the tile %d", p.nx, p.ny, p.tile_x);
+        if (p.bc_x != DBA_BC_PERIODIC) return fail(DBA_ERR_INVALID, "KH needs periodic x");
+        if (p.bc_y != DBA_BC_PERIODIC && p.bc_y != DBA_BC_WALL) return fail(DBA_ERR_INVALID, "KH y: periodic or wall");
+        if (p.dt_mode == DBA_DT_CFL && p.mode == DBA_MODE_CONSERVED)
+            return fail(DBA_ERR_INVALID, "conserved KH uses a fixed dt");
+        if (p.dt_mode == DBA_DT_CFL && !(p.cfl > 0.0)) return fail(DBA_ERR_INVALID, "cfl must be positive");
+        if (!(p.gas_gamma > 1.0)) return fail(DBA_ERR_INVALID, "gas_gamma must exceed 1");
+        if (p.dy != p.dx) return fail(DBA_ERR_INVALID, "KH needs dy == dx");
+    }
     if (p.model == DBA_MODEL_CH && p.stencil != 5 && p.stencil != 9)
         return fail(DBA_ERR_INVALID, "CH stencil must be 5 or 9");
     if (p.model == DBA_MODEL_CH && p.mode == DBA_MODE_CONSERVED && p.bc_x != DBA_BC_PERIODIC)
@@ -175,7 +192,11 @@ Layout layout_of(const dba_params& p) {
     L.n = (int64_t)p.nx * p.ny * p.nz;
     L.nf = nfields_of(p.model);
     L.nsets = nsets_of(p.model);
-    L.naux = (p.model == DBA_MODEL_CH && p.mode != DBA_MODE_REGULAR) ? 1 : 0;
+    L.naux = 0;
+    if (p.mode != DBA_MODE_REGULAR) {
+        if (p.model == DBA_MODEL_CH) L.naux = 1;   // mu ring
+        if (p.model == DBA_MODEL_KH) L.naux = 24;  // edge state of sleeping tiles
+    }
     return L;
 }
 
@@ -246,6 +267,21 @@ int default_threads(int model, int tile) {
     return 256;
 }
 
+// KH: wave speeds of every tile from the current state, and the first CFL dt.
+dba_status kh_prime(dba_ctx* ctx, Instance* in) {
+    if (in->p.tile_x == 8)
+        hipLaunchKernelGGL((kh_prime_kernel<8>), dim3(std::min(in->h.nt, 4096)), dim3(256), 0, ctx->stream,
+                           in->d_self, 0);
+    else
+        hipLaunchKernelGGL((kh_prime_kernel<16>), dim3(std::min(in->h.nt, 4096)), dim3(256), 0, ctx->stream,
+                           in->d_self, 0);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(kh_dt_kernel, dim3(1), dim3(256), 0, ctx->stream, in->d_self, 0);
+    LAUNCH_CHECK();
+    in->needs_prime = false;
+    return DBA_OK;
+}
+
 dba_status get_group(dba_ctx* ctx, const std::vector<int>& ids, Group** out) {
     auto it = ctx->groups.find(ids);
     if (it != ctx->groups.end()) {
@@ -312,6 +348,12 @@ void launch_step_tb(const Group* g, hipStream_t s) {
         case DBA_MODEL_DENDRITE_K93:
             hipLaunchKernelGGL((k93_step_kernel<TB, NT>), grid, block, 0, s, g->d_insts, g->d_segs, g->d_offsets,
                                g->nseg);
+            break;
+        case DBA_MODEL_KH:
+            if constexpr (TB <= 16) {
+                hipLaunchKernelGGL((kh_step_kernel<TB, NT>), grid, block, 0, s, g->d_insts, g->d_segs, g->d_offsets,
+                                   g->nseg);
+            }
             break;
         default:
             break;
@@ -414,6 +456,14 @@ dba_status prepare_groups(dba_ctx* ctx, const int32_t* ids, int32_t n_ids, std::
         if (st != DBA_OK) return st;
         gs->push_back(g);
     }
+    // KH instances whose state was uploaded since their last step
+    for (int k = 0; k < n_ids; k++) {
+        Instance* in = ctx->inst[ids[k]];
+        if (in->needs_prime) {
+            st = kh_prime(ctx, in);
+            if (st != DBA_OK) return st;
+        }
+    }
     // (re)build the worklists from the current awake sets
     for (Group* g : *gs) {
         launch_finalize(g, ctx->stream, 0);
@@ -512,7 +562,8 @@ dba_status dba_instance_create(dba_ctx* ctx, const dba_params* pp, int32_t* id) 
     h.stencil = p.stencil; h.aniso_j = p.aniso_j; h.dt_mode = p.dt_mode; h.metric_kind = p.metric_kind;
     h.nsets = L.nsets;
     h.nfields = L.nf;
-    h.ncons = (p.mode == DBA_MODE_CONSERVED) ? (p.model == DBA_MODEL_CH ? 1 : 0) : 0;
+    h.ncons = 0;
+    if (p.mode == DBA_MODE_CONSERVED) h.ncons = p.model == DBA_MODEL_CH ? 1 : (p.model == DBA_MODEL_KH ? 4 : 0);
     for (int f = 0; f < kMaxFields; f++) h.field_set[f] = 0;
     if (L.nsets == 2) h.field_set[1] = 1;
     h.full_tiles = (p.nx % p.tile_x == 0 && p.ny % p.tile_y == 0 && p.nz % p.tile_z == 0) ? 1 : 0;
@@ -623,6 +674,7 @@ dba_status dba_upload(dba_ctx* ctx, int32_t id, int32_t field, const double* hos
         hipLaunchKernelGGL(reset_epochs_kernel, dim3(64), dim3(256), 0, ctx->stream, in->d_self, 0);
         LAUNCH_CHECK();
     }
+    if (in->p.model == DBA_MODEL_KH) in->needs_prime = true;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return DBA_OK;
 }

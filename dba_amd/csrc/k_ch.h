@@ -68,10 +68,10 @@ __device__ __forceinline__ void ch_tile(const Inst& I, int t, NbTable& nb, doubl
     double* __restrict__ mu_store = I.aux[0];
     const double kappa = I.kappa, eps = I.thresh_a;
     const double inv = NINE ? I.inv6dx2 : I.invdx2;
-    nb_fill<TB>(nb, I, t, x0, y0, regular, 1, cons, 0);
+    nb_fill<TB>(nb, I, t, x0, y0, regular, 1, cons ? 1 : 0);
     __syncthreads();
     const int pself = nb.par[0][4];
-    const double sh_self = nb.shift[4];
+    const double sh_self = nb.shift[0][4];
     const double* __restrict__ cur = pself ? b1 : b0;
     double* __restrict__ nxt = pself ? I.buf[0][0] : I.buf[1][0];
 
@@ -90,7 +90,7 @@ __device__ __forceinline__ void ch_tile(const Inst& I, int t, NbTable& nb, doubl
         else { const int q = k - 4 * W2; ly = 2 + q / 4; const int c4 = q % 4; lx = c4 < 2 ? c4 : TB + c4; }
         const int d = nb_dir1<TB, 2>(lx) + 3 * nb_dir1<TB, 2>(ly);
         const int mx = map_index(x0 + lx - 2, nx, bcx), my = map_index(y0 + ly - 2, ny, bcy);
-        sc[lx + ly * W2] = (nb.par[0][d] ? b1 : b0)[mx + (int64_t)my * nx] - nb.shift[d];
+        sc[lx + ly * W2] = (nb.par[0][d] ? b1 : b0)[mx + (int64_t)my * nx] - nb.shift[0][d];
     }
     __syncthreads();
 

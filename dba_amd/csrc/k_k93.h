@@ -70,7 +70,7 @@ __global__ __launch_bounds__(NT) void k93_step_kernel(const Inst* __restrict__ i
         const double* __restrict__ p1 = I.buf[1][0];
         const double* __restrict__ T0 = I.buf[0][1];
         const double* __restrict__ T1 = I.buf[1][1];
-        nb_fill<TB>(nb, I, t, x0, y0, regular, 2, false, 0);
+        nb_fill<TB>(nb, I, t, x0, y0, regular, 2, 0);
         __syncthreads();
         const int ap = nb.act[0][4], at = nb.act[1][4];
         const int pp0 = nb.par[0][4], pp1 = nb.par[1][4];

@@ -1,0 +1,402 @@
+// k_kh.h - fused DBA step for 2D compressible Euler (Kelvin-Helmholtz):
+// MUSCL-Hancock with central gradients, half-step predictor and Rusanov fluxes.
+//
+// Reference counterpart: getPrimitive_wakeup -> extrap_in_time_wakeup ->
+// calc_flux_wakeup -> apply_flux_wakeup -> Wakeup_Next
+// (example_Kelvin_Helmholtz/KH_DBA.cu:418-985), conserved variant
+// example_Kelvin_Helmholtz/Conserved/KH_DBA.cu:832-955. Same arithmetic
+// (getFlux :662-697, extrapolateInSpaceToFace :630-660, applyFluxes :818-841),
+// generalised to a gas gamma, a CFL time step and reflective y walls (mirrored
+// ghost cells with the normal velocity negated).
+//
+// The reference reads the primitives, gradients, primes and fluxes that
+// sleeping neighbours left behind. Only a tile's edges are ever read that way,
+// so each awake tile stores exactly those: primitives on its ring, (primes,
+// x-gradients) on its left column, (primes, y-gradients) on its bottom row, the
+// x-face flux of its right column and the y-face flux of its top row.
+// Everything else is recomputed in LDS from the conserved state of the tile
+// and its neighbours (2-cell halo).
+//
+// LDS (TB = 16): primitives 4 x 20^2, primes / x- / y-gradients 12 x 18^2;
+// the x-face fluxes reuse the primitive space, the y-face fluxes the
+// x-gradient space.
+#pragma once
+#include "common.h"
+
+namespace dba {
+
+struct KhC {
+    double g, dx, dy, dt, vol;
+};
+
+__device__ __forceinline__ void kh_prims(const KhC& c, double m, double mx, double my, double e, double* r,
+                                         double* u, double* v, double* p) {
+    const double rho = m / c.vol;
+    const double uu = mx / rho / c.vol;
+    const double vv = my / rho / c.vol;
+    *r = rho;
+    *u = uu;
+    *v = vv;
+    *p = (e / c.vol - 0.5 * rho * (uu * uu + vv * vv)) * (c.g - 1.0);
+}
+
+__device__ __forceinline__ double kh_speed(const KhC& c, double r, double u, double v, double p) {
+    return sqrt(c.g * p / r) + sqrt(u * u + v * v);
+}
+
+// getFlux (example_Kelvin_Helmholtz/KH_DBA.cu:662-697)
+__device__ __forceinline__ void kh_rusanov(const KhC& c, double rL, double rR, double pL, double pR, double uL,
+                                           double uR, double vL, double vR, double* fm, double* fe, double* fu,
+                                           double* fv) {
+    const double g = c.g;
+    const double en_L = pL / (g - 1.0) + 0.5 * rL * (uL * uL + vL * vL);
+    const double en_R = pR / (g - 1.0) + 0.5 * rR * (uR * uR + vR * vR);
+    const double rho_star = 0.5 * (rL + rR);
+    const double momx_star = 0.5 * (rL * uL + rR * uR);
+    const double momy_star = 0.5 * (rL * vL + rR * vR);
+    const double en_star = 0.5 * (en_L + en_R);
+    const double p_star = (g - 1.0) * (en_star - 0.5 * (momx_star * momx_star + momy_star * momy_star) / rho_star);
+    const double flux_Mass = momx_star;
+    const double flux_Momx = momx_star * momx_star / rho_star + p_star;
+    const double flux_Momy = momx_star * momy_star / rho_star;
+    const double flux_Energy = (en_star + p_star) * momx_star / rho_star;
+    const double C_L = sqrt(g * pL / rL) + fabs(uL);
+    const double C_R = sqrt(g * pR / rR) + fabs(uR);
+    const double C = (C_L >= C_R) ? C_L : C_R;
+    *fm = flux_Mass - C * 0.5 * (rL - rR);
+    *fu = flux_Momx - C * 0.5 * (rL * uL - rR * uR);
+    *fv = flux_Momy - C * 0.5 * (rL * vL - rR * vR);
+    *fe = flux_Energy - C * 0.5 * (en_L - en_R);
+}
+
+template <int TB, int NT>
+__global__ __launch_bounds__(NT) void kh_step_kernel(const Inst* __restrict__ insts, const Seg* __restrict__ segs,
+                                                    const int32_t* __restrict__ offsets, int nseg) {
+    constexpr int W2 = TB + 4;
+    constexpr int W1 = TB + 2;
+    constexpr int A2 = W2 * W2;
+    constexpr int A1 = W1 * W1;
+    static_assert(4 * A2 >= 4 * (TB + 1) * TB && 4 * A1 >= 4 * (TB + 1) * TB, "flux reuse");
+    __shared__ double sP[4 * A2];    // primitives r, vx, vy, p; later x-face fluxes
+    __shared__ double sPR[4 * A1];   // primes
+    __shared__ double sGX[4 * A1];   // x-gradients; later y-face fluxes
+    __shared__ double sGY[4 * A1];   // y-gradients
+    __shared__ double sred[NT / 64 + 1];
+    __shared__ int sflag[NT / 64 + 1];
+    __shared__ NbTable nb;
+
+    const int total = offsets[nseg];
+    const int G = gridDim.x;
+    const int tid = threadIdx.x;
+    for (int item = xcd_remap(blockIdx.x, G); item < total; item += G) {
+        const int s = nseg == 1 ? 0 : find_slot(offsets, nseg, item);
+        const Seg sg = segs[s];
+        const Inst& I = insts[sg.inst];
+        const int t = I.seg[sg.tile0 + (item - offsets[s])];
+        const int nx = I.nx, ny = I.ny, ntx = I.ntx;
+        const int x0 = (t % ntx) * TB, y0 = (t / ntx) * TB;
+        const bool regular = I.mode == 0;
+        const bool dba = !regular;
+        const int ncons = I.ncons;
+        const bool wall = I.bc_y == 2;
+        const bool cfl = I.dt_mode == 1;
+        KhC c;
+        c.g = I.gas_gamma;
+        c.dx = I.dx;
+        c.dy = I.dy;
+        c.vol = I.vol;
+        c.dt = cfl ? I.scal[1] : I.dt;
+        nb_fill<TB>(nb, I, t, x0, y0, regular, 1, ncons);
+        __syncthreads();
+        const int pself = nb.par[0][4];
+
+        // 1. primitives on the tile + 2-cell halo: from U where the owner runs
+        //    this step, else the primitives its ring kept
+        for (int k = tid; k < A2; k += NT) {
+            const int lx = k % W2, ly = k / W2;
+            const int d = nb_dir1<TB, 2>(lx) + 3 * nb_dir1<TB, 2>(ly);
+            const int gy = y0 + ly - 2;
+            const int mx = map_index(x0 + lx - 2, nx, 0), my = map_index(gy, ny, wall ? 1 : 0);
+            const int64_t gi = mx + (int64_t)my * nx;
+            const double sgn = (wall && (gy < 0 || gy >= ny)) ? -1.0 : 1.0;
+            double r, u, v, p;
+            if (nb.act[0][d]) {
+                const int pp = nb.par[0][d];
+                const double um = I.buf[pp][0][gi] - nb.shift[0][d];
+                const double ux = I.buf[pp][1][gi] - nb.shift[1][d];
+                const double uy = I.buf[pp][2][gi] - nb.shift[2][d];
+                const double ue = I.buf[pp][3][gi] - nb.shift[3][d];
+                kh_prims(c, um, ux, uy, ue, &r, &u, &v, &p);
+            } else {
+                r = I.aux[0][gi];
+                u = I.aux[1][gi];
+                v = I.aux[2][gi];
+                p = I.aux[3][gi];
+            }
+            sP[0 * A2 + k] = r;
+            sP[1 * A2 + k] = u;
+            sP[2 * A2 + k] = sgn * v;
+            sP[3 * A2 + k] = p;
+        }
+        __syncthreads();
+
+        // stale speed (primitives at the start of the step) + the tile's ring primitives
+        double vstale = 0.0;
+        for (int k = tid; k < TB * TB; k += NT) {
+            const int lx = k % TB, ly = k / TB;
+            const int q = (lx + 2) + (ly + 2) * W2;
+            const double r = sP[q], u = sP[A2 + q], v = sP[2 * A2 + q], p = sP[3 * A2 + q];
+            if (cfl) vstale = fmax(vstale, kh_speed(c, r, u, v, p));
+            if (dba && (lx == 0 || lx == TB - 1 || ly == 0 || ly == TB - 1)) {
+                const int64_t gi = (x0 + lx) + (int64_t)(y0 + ly) * nx;
+                I.aux[0][gi] = r;
+                I.aux[1][gi] = u;
+                I.aux[2][gi] = v;
+                I.aux[3][gi] = p;
+            }
+        }
+
+        // 2. gradients, primes on the tile + ring (no ring corners); metric on the tile
+        int flag = 0;
+        for (int k = tid; k < A1; k += NT) {
+            const int lx = k % W1, ly = k / W1;
+            const bool cx = lx == 0 || lx == TB + 1, cy = ly == 0 || ly == TB + 1;
+            if (cx && cy) continue;
+            const int d = nb_dir1<TB, 1>(lx) + 3 * nb_dir1<TB, 1>(ly);
+            if (!nb.act[0][d]) {
+                // sleeping neighbour: its stored edge state (right: primes + x-grads
+                // of its left column; top: primes + y-grads of its bottom row)
+                if (lx == TB + 1 || ly == TB + 1) {
+                    const int mx = map_index(x0 + lx - 1, nx, 0), my = map_index(y0 + ly - 1, ny, wall ? 1 : 0);
+                    const int64_t gi = mx + (int64_t)my * nx;
+                    for (int q = 0; q < 4; q++) {
+                        sPR[q * A1 + k] = I.aux[4 + q][gi];
+                        if (lx == TB + 1) sGX[q * A1 + k] = I.aux[8 + q][gi];
+                        else sGY[q * A1 + k] = I.aux[12 + q][gi];
+                    }
+                }
+                continue;
+            }
+            const int p0 = (lx + 1) + (ly + 1) * W2;
+            const int pxm = p0 - 1, pxp = p0 + 1, pym = p0 - W2, pyp = p0 + W2;
+            // Grad_x / Grad_y (example_Kelvin_Helmholtz/KH_DBA.cu:249-275)
+            const double r_x = (sP[pxp] - sP[pxm]) / 2.0 / c.dx, r_y = (sP[pyp] - sP[pym]) / 2.0 / c.dy;
+            const double u_x = (sP[A2 + pxp] - sP[A2 + pxm]) / 2.0 / c.dx;
+            const double u_y = (sP[A2 + pyp] - sP[A2 + pym]) / 2.0 / c.dy;
+            const double v_x = (sP[2 * A2 + pxp] - sP[2 * A2 + pxm]) / 2.0 / c.dx;
+            const double v_y = (sP[2 * A2 + pyp] - sP[2 * A2 + pym]) / 2.0 / c.dy;
+            const double p_x = (sP[3 * A2 + pxp] - sP[3 * A2 + pxm]) / 2.0 / c.dx;
+            const double p_y = (sP[3 * A2 + pyp] - sP[3 * A2 + pym]) / 2.0 / c.dy;
+            const double r = sP[p0], u = sP[A2 + p0], v = sP[2 * A2 + p0], p = sP[3 * A2 + p0];
+            const double dt = c.dt, g = c.g;
+            // extrap_in_time (example_Kelvin_Helmholtz/KH_DBA.cu:524-531)
+            const double rp = r - 0.5 * dt * (u * r_x + r * u_x + v * r_y + r * v_y);
+            const double up = u - 0.5 * dt * (u * u_x + v * u_y + (1.0 / r) * p_x);
+            const double vp = v - 0.5 * dt * (u * v_x + v * v_y + (1.0 / r) * p_y);
+            const double pp = p - 0.5 * dt * (g * p * (u_x + v_y) + u * p_x + v * p_y);
+            sPR[k] = rp;
+            sPR[A1 + k] = up;
+            sPR[2 * A1 + k] = vp;
+            sPR[3 * A1 + k] = pp;
+            sGX[k] = r_x;
+            sGX[A1 + k] = u_x;
+            sGX[2 * A1 + k] = v_x;
+            sGX[3 * A1 + k] = p_x;
+            sGY[k] = r_y;
+            sGY[A1 + k] = u_y;
+            sGY[2 * A1 + k] = v_y;
+            sGY[3 * A1 + k] = p_y;
+            if (d == 4) {
+                const double metric = I.metric_kind == 1
+                    ? fabs(r_x) + fabs(r_y)
+                    : fabs(r_x) + fabs(r_y) + fabs(p_x) + fabs(p_y) + fabs(u_x) + fabs(u_y) + fabs(v_x) + fabs(v_y);
+                flag |= metric > I.thresh_a;
+                if (dba && (lx == 1 || ly == 1)) {
+                    const int64_t gi = (x0 + lx - 1) + (int64_t)(y0 + ly - 1) * nx;
+                    I.aux[4][gi] = rp;
+                    I.aux[5][gi] = up;
+                    I.aux[6][gi] = vp;
+                    I.aux[7][gi] = pp;
+                    if (lx == 1) {
+                        I.aux[8][gi] = r_x;
+                        I.aux[9][gi] = u_x;
+                        I.aux[10][gi] = v_x;
+                        I.aux[11][gi] = p_x;
+                    }
+                    if (ly == 1) {
+                        I.aux[12][gi] = r_y;
+                        I.aux[13][gi] = u_y;
+                        I.aux[14][gi] = v_y;
+                        I.aux[15][gi] = p_y;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+
+        // 3a. x-face fluxes of W1 columns 0..TB on the tile rows (into the primitive space)
+        constexpr int FW = TB + 1;  // flux array row width (x faces) / rows (y faces)
+        for (int k = tid; k < FW * TB; k += NT) {
+            const int lx = k % FW, ly = 1 + k / FW;  // W1 coordinates of the cell left of the face
+            const int a = lx + ly * W1, b = a + 1;
+            double fm, fe, fu, fv;
+            const int d = nb_dir1<TB, 1>(lx) + 3 * nb_dir1<TB, 1>(ly);
+            if (lx == 0 && !nb.act[0][d]) {
+                const int mx = map_index(x0 - 1, nx, 0);
+                const int64_t gi = mx + (int64_t)(y0 + ly - 1) * nx;
+                fm = I.aux[16][gi];
+                fu = I.aux[17][gi];
+                fv = I.aux[18][gi];
+                fe = I.aux[19][gi];
+            } else {
+                double L[4], R[4];
+                for (int q = 0; q < 4; q++) {
+                    L[q] = sPR[q * A1 + b] - sGX[q * A1 + b] * c.dx / 2.0;  // face[0] of the right cell
+                    R[q] = sPR[q * A1 + a] + sGX[q * A1 + a] * c.dx / 2.0;  // face[1] of the left cell
+                }
+                kh_rusanov(c, L[0], R[0], L[3], R[3], L[1], R[1], L[2], R[2], &fm, &fe, &fu, &fv);
+                if (dba && lx == TB) {
+                    const int64_t gi = (x0 + TB - 1) + (int64_t)(y0 + ly - 1) * nx;
+                    I.aux[16][gi] = fm;
+                    I.aux[17][gi] = fu;
+                    I.aux[18][gi] = fv;
+                    I.aux[19][gi] = fe;
+                }
+            }
+            sP[0 * A2 + k] = fm;
+            sP[1 * A2 + k] = fu;
+            sP[2 * A2 + k] = fv;
+            sP[3 * A2 + k] = fe;
+        }
+        // 3b. y-face fluxes of W1 rows 0..TB on the tile columns need sGX no longer
+        //     only after 3a has read it
+        __syncthreads();
+        for (int k = tid; k < FW * TB; k += NT) {
+            const int ly = k / TB, lx = 1 + k % TB;  // W1 coordinates of the cell below the face
+            const int a = lx + ly * W1, b = a + W1;
+            double fm, fe, fu, fv;
+            const int d = nb_dir1<TB, 1>(lx) + 3 * nb_dir1<TB, 1>(ly);
+            if (ly == 0 && !nb.act[0][d]) {
+                const int my = map_index(y0 - 1, ny, wall ? 1 : 0);
+                const int64_t gi = (x0 + lx - 1) + (int64_t)my * nx;
+                fm = I.aux[20][gi];
+                fu = I.aux[21][gi];
+                fv = I.aux[22][gi];
+                fe = I.aux[23][gi];
+            } else {
+                double L[4], R[4];
+                for (int q = 0; q < 4; q++) {
+                    L[q] = sPR[q * A1 + b] - sGY[q * A1 + b] * c.dy / 2.0;  // face[2] of the upper cell
+                    R[q] = sPR[q * A1 + a] + sGY[q * A1 + a] * c.dy / 2.0;  // face[3] of the lower cell
+                }
+                // momentum roles swapped for y faces (example_Kelvin_Helmholtz/KH_DBA.cu:738-740)
+                kh_rusanov(c, L[0], R[0], L[3], R[3], L[2], R[2], L[1], R[1], &fm, &fe, &fv, &fu);
+                if (dba && ly == TB) {
+                    const int64_t gi = (x0 + lx - 1) + (int64_t)(y0 + TB - 1) * nx;
+                    I.aux[20][gi] = fm;
+                    I.aux[21][gi] = fu;
+                    I.aux[22][gi] = fv;
+                    I.aux[23][gi] = fe;
+                }
+            }
+            sGX[0 * A1 + k] = fm;
+            sGX[1 * A1 + k] = fu;
+            sGX[2 * A1 + k] = fv;
+            sGX[3 * A1 + k] = fe;
+        }
+        __syncthreads();
+
+        // 4. apply the fluxes (applyFluxes, example_Kelvin_Helmholtz/KH_DBA.cu:839-840)
+        const double dtdx = c.dt * c.dx;
+        double part[4] = {0.0, 0.0, 0.0, 0.0};
+        double vfresh = 0.0;
+        for (int k = tid; k < TB * TB; k += NT) {
+            const int lx = k % TB, ly = k / TB;
+            const int64_t gi = (x0 + lx) + (int64_t)(y0 + ly) * nx;
+            const int fx = (lx + 1) + ly * FW, fxm = fx - 1;         // x-face arrays: (TB+1) x TB
+            const int fy = lx + (ly + 1) * TB, fym = fy - TB;        // y-face arrays: TB x (TB+1)
+            double un[4];
+            for (int q = 0; q < 4; q++) {
+                const double ch = dtdx * (-1.0 * sP[q * A2 + fx] + sP[q * A2 + fxm] - 1.0 * sGX[q * A1 + fy] +
+                                          sGX[q * A1 + fym]);
+                const double u0 = I.buf[pself][q][gi] - nb.shift[q][4];
+                un[q] = u0 + ch;
+                part[q] += ch;
+                I.buf[pself ^ 1][q][gi] = un[q];
+            }
+            if (cfl) {
+                double r, u, v, p;
+                kh_prims(c, un[0], un[1], un[2], un[3], &r, &u, &v, &p);
+                vfresh = fmax(vfresh, kh_speed(c, r, u, v, p));
+            }
+        }
+
+        // 5. detector, per-tile speeds, conserved bookkeeping
+        if (dba) {
+            const int any = block_any<NT>(flag, sflag);
+            if (tid == 0) I.nflag[0][t] = (uint8_t)any;
+        }
+        if (cfl) {
+            const double vs = block_max<NT>(vstale, sred);
+            const double vf = block_max<NT>(vfresh, sred);
+            if (tid == 0) {
+                I.tspeed[t] = vf;
+                I.tspeed[I.nt + t] = vs;
+            }
+        }
+        for (int q = 0; q < ncons; q++) {
+            const double tot = block_sum<NT>(part[q], sred);
+            if (tid == 0) {
+                I.tpart[(int64_t)q * I.nt + t] = tot;
+                I.s_tile[((int64_t)q * 2 + (pself ^ 1)) * I.nt + t] = I.scal[8 + q];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Per-tile wave speeds of the current state (all tiles), so the first step of
+// a CFL run has its dt; the finalize then keeps them current.
+template <int TB>
+__global__ void kh_prime_kernel(const Inst* __restrict__ insts, int slot) {
+    const Inst& I = insts[slot];
+    KhC c;
+    c.g = I.gas_gamma;
+    c.dx = I.dx;
+    c.dy = I.dy;
+    c.vol = I.vol;
+    c.dt = 0.0;
+    __shared__ double sred[5];
+    for (int t = blockIdx.x; t < I.nt; t += gridDim.x) {
+        const int x0 = (t % I.ntx) * TB, y0 = (t / I.ntx) * TB;
+        const int pp = bit_of(I.par[0], t);
+        double vm = 0.0;
+        for (int k = threadIdx.x; k < TB * TB; k += blockDim.x) {
+            const int64_t gi = (x0 + k % TB) + (int64_t)(y0 + k / TB) * I.nx;
+            double sh[4];
+            for (int q = 0; q < 4; q++)
+                sh[q] = q < I.ncons ? I.scal[8 + q] - I.s_tile[((int64_t)q * 2 + pp) * I.nt + t] : 0.0;
+            double r, u, v, p;
+            kh_prims(c, I.buf[pp][0][gi] - sh[0], I.buf[pp][1][gi] - sh[1], I.buf[pp][2][gi] - sh[2],
+                     I.buf[pp][3][gi] - sh[3], &r, &u, &v, &p);
+            vm = fmax(vm, kh_speed(c, r, u, v, p));
+        }
+        vm = block_max<256>(vm, sred);
+        if (threadIdx.x == 0) {
+            I.tspeed[t] = vm;
+            I.tspeed[I.nt + t] = vm;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void kh_dt_kernel(const Inst* __restrict__ insts, int slot) {
+    const Inst& I = insts[slot];
+    __shared__ double sred[5];
+    double vm = 0.0;
+    for (int t = threadIdx.x; t < I.nt; t += blockDim.x) vm = fmax(vm, I.tspeed[t]);
+    vm = block_max<256>(vm, sred);
+    if (threadIdx.x == 0 && I.dt_mode == 1) I.scal[1] = I.cfl * (I.dx / vm);
+}
+
+}  // namespace dba

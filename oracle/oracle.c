@@ -39,6 +39,7 @@ struct ora_sim {
     int hist_n, hist_cap;
     long steps;
     double time, last_dt;
+    double dt_kh;        /* KH: time step of the current step (fixed or CFL) */
     /* derived constants (identical expressions in the HIP host code) */
     double invdx2, inv6dx2, inv2dx, dt_tau, alpha_pi, cos_jt0, sin_jt0, k_epsp;
 };
@@ -335,6 +336,235 @@ static void k93_step(ora_sim* s) {
     dilate_set(s, 1);
 }
 
+
+/* ---------------------------------------------------------------- KH model */
+/* 2D compressible Euler, MUSCL-Hancock with Rusanov fluxes, in the arithmetic
+   of example_Kelvin_Helmholtz/CPU_version/KH_CPU.cpp:263-515 (getPrimitive,
+   extrap_in_time, extrapolateInSpaceToFace, getFlux, applyFluxes) and the DBA
+   step of example_Kelvin_Helmholtz/KH_DBA.cu:1436-1500 (prims, gradients,
+   primes and fluxes of sleeping blocks stay stale and are read as such).
+   Generalised: gas gamma, CFL time step (example_Kelvin_Helmholtz/CPU_version/KH.py:241),
+   reflective y walls through mirrored ghost cells (normal velocity negated),
+   density-gradient detector (example_Kelvin_Helmholtz/KH_DBA.cu:618).
+   Primitive, gradient, prime and flux arrays carry a 2-cell ghost frame.
+   Field order: U = Mass, Momx, Momy, Energy; prims r, vx, vy, p;
+   grads r_x r_y p_x p_y vx_x vx_y vy_x vy_y (reference order). */
+
+enum { KP_R = 0, KP_VX = 1, KP_VY = 2, KP_P = 3 };
+
+static inline long pidx(const ora_sim* s, int x, int y) { /* padded index, x,y in [-2, n+1] */
+    return (long)(x + 2) + (long)(y + 2) * (s->nx + 4);
+}
+
+/* ghost source: mapped real cell and sign for vy (wall mirror in y) */
+static inline void kh_map(const ora_sim* s, int x, int y, int* mx, int* my, double* sgn) {
+    *mx = map_i(x, s->nx, s->p.bc_x);
+    *my = map_i(y, s->ny, s->p.bc_y == DBA_BC_WALL ? DBA_BC_NOFLUX : s->p.bc_y);
+    *sgn = (s->p.bc_y == DBA_BC_WALL && (y < 0 || y >= s->ny)) ? -1.0 : 1.0;
+}
+
+static void kh_prims_of(const ora_sim* s, long i, double* r, double* vx, double* vy, double* pr) {
+    double vol = s->p.dx * s->p.dy, g = s->p.gas_gamma;
+    double rho = s->f[0][i] / vol;
+    double u = s->f[1][i] / rho / vol;
+    double v = s->f[2][i] / rho / vol;
+    *r = rho;
+    *vx = u;
+    *vy = v;
+    *pr = (s->f[3][i] / vol - 0.5 * rho * (u * u + v * v)) * (g - 1.0);
+}
+
+static inline double kh_speed(const ora_sim* s, double r, double u, double v, double pr) {
+    return sqrt(s->p.gas_gamma * pr / r) + sqrt(u * u + v * v);
+}
+
+static void kh_flux(const ora_sim* s, double rL, double rR, double pL, double pR, double uL, double uR, double vL,
+                    double vR, double* fm, double* fe, double* fu, double* fv) {
+    /* getFlux, example_Kelvin_Helmholtz/KH_DBA.cu:662-697 */
+    double g = s->p.gas_gamma;
+    double en_L = pL / (g - 1.0) + 0.5 * rL * (uL * uL + vL * vL);
+    double en_R = pR / (g - 1.0) + 0.5 * rR * (uR * uR + vR * vR);
+    double rho_star = 0.5 * (rL + rR);
+    double momx_star = 0.5 * (rL * uL + rR * uR);
+    double momy_star = 0.5 * (rL * vL + rR * vR);
+    double en_star = 0.5 * (en_L + en_R);
+    double p_star = (g - 1.0) * (en_star - 0.5 * (momx_star * momx_star + momy_star * momy_star) / rho_star);
+    double flux_Mass = momx_star;
+    double flux_Momx = momx_star * momx_star / rho_star + p_star;
+    double flux_Momy = momx_star * momy_star / rho_star;
+    double flux_Energy = (en_star + p_star) * momx_star / rho_star;
+    double C_L = sqrt(g * pL / rL) + fabs(uL);
+    double C_R = sqrt(g * pR / rR) + fabs(uR);
+    double C = (C_L >= C_R) ? C_L : C_R;
+    *fm = flux_Mass - C * 0.5 * (rL - rR);
+    *fu = flux_Momx - C * 0.5 * (rL * uL - rR * uR);
+    *fv = flux_Momy - C * 0.5 * (rL * vL - rR * vR);
+    *fe = flux_Energy - C * 0.5 * (en_L - en_R);
+}
+
+/* w[0..3] prims, w[4..11] grads, w[12..15] primes, w[16..19] F (x faces),
+   w[20..23] G (y faces), w[24..27] conserved changes, w[28] metric. */
+static void kh_grads_at(ora_sim* s, int x, int y) {
+    long c = pidx(s, x, y);
+    long xm = pidx(s, x - 1, y), xp = pidx(s, x + 1, y), ym = pidx(s, x, y - 1), yp = pidx(s, x, y + 1);
+    double dx = s->p.dx, dy = s->p.dy, dt = s->dt_kh;
+    double** P = &s->w[0];
+    /* Grad_x / Grad_y, example_Kelvin_Helmholtz/KH_DBA.cu:249-275 */
+    double r_x = (P[KP_R][xp] - P[KP_R][xm]) / 2.0 / dx, r_y = (P[KP_R][yp] - P[KP_R][ym]) / 2.0 / dy;
+    double p_x = (P[KP_P][xp] - P[KP_P][xm]) / 2.0 / dx, p_y = (P[KP_P][yp] - P[KP_P][ym]) / 2.0 / dy;
+    double u_x = (P[KP_VX][xp] - P[KP_VX][xm]) / 2.0 / dx, u_y = (P[KP_VX][yp] - P[KP_VX][ym]) / 2.0 / dy;
+    double v_x = (P[KP_VY][xp] - P[KP_VY][xm]) / 2.0 / dx, v_y = (P[KP_VY][yp] - P[KP_VY][ym]) / 2.0 / dy;
+    double r = P[KP_R][c], u = P[KP_VX][c], v = P[KP_VY][c], pr = P[KP_P][c];
+    double g = s->p.gas_gamma;
+    s->w[4][c] = r_x; s->w[5][c] = r_y; s->w[6][c] = p_x; s->w[7][c] = p_y;
+    s->w[8][c] = u_x; s->w[9][c] = u_y; s->w[10][c] = v_x; s->w[11][c] = v_y;
+    /* extrap_in_time, example_Kelvin_Helmholtz/KH_DBA.cu:524-531 */
+    s->w[12][c] = r - 0.5 * dt * (u * r_x + r * u_x + v * r_y + r * v_y);
+    s->w[13][c] = u - 0.5 * dt * (u * u_x + v * u_y + (1.0 / r) * p_x);
+    s->w[14][c] = v - 0.5 * dt * (u * v_x + v * v_y + (1.0 / r) * p_y);
+    s->w[15][c] = pr - 0.5 * dt * (g * pr * (u_x + v_y) + u * p_x + v * p_y);
+    if (s->p.metric_kind == DBA_KH_METRIC_DENSITY)
+        s->w[28][c] = fabs(r_x) + fabs(r_y);
+    else
+        s->w[28][c] = fabs(r_x) + fabs(r_y) + fabs(p_x) + fabs(p_y) + fabs(u_x) + fabs(u_y) + fabs(v_x) + fabs(v_y);
+}
+
+/* x-face flux between (x,y) and (x+1,y) stored at (x,y); y-face between (x,y) and (x,y+1) */
+static void kh_face_fluxes(ora_sim* s, int x, int y, int do_x, int do_y) {
+    long c = pidx(s, x, y), xp = pidx(s, x + 1, y), yp = pidx(s, x, y + 1);
+    double dx = s->p.dx, dy = s->p.dy;
+    double* PR[4] = {s->w[12], s->w[13], s->w[14], s->w[15]};            /* r, vx, vy, p primes */
+    double* GX[4] = {s->w[4], s->w[8], s->w[10], s->w[6]};               /* r_x, vx_x, vy_x, p_x */
+    double* GY[4] = {s->w[5], s->w[9], s->w[11], s->w[7]};
+    double f0[4], f1[4], f2[4], f3[4];
+    for (int q = 0; q < 4; q++) {
+        /* extrapolateInSpaceToFace, example_Kelvin_Helmholtz/KH_DBA.cu:656-659 */
+        f0[q] = PR[q][xp] - GX[q][xp] * dx / 2.0;
+        f1[q] = PR[q][c] + GX[q][c] * dx / 2.0;
+        f2[q] = PR[q][yp] - GY[q][yp] * dy / 2.0;
+        f3[q] = PR[q][c] + GY[q][c] * dy / 2.0;
+    }
+    double fm, fe, fu, fv;
+    if (do_x) {
+        kh_flux(s, f0[0], f1[0], f0[3], f1[3], f0[1], f1[1], f0[2], f1[2], &fm, &fe, &fu, &fv);
+        s->w[16][c] = fm; s->w[17][c] = fu; s->w[18][c] = fv; s->w[19][c] = fe;
+    }
+    if (do_y) {
+        /* momentum roles swapped for the y face, example_Kelvin_Helmholtz/KH_DBA.cu:738-740 */
+        kh_flux(s, f2[0], f3[0], f2[3], f3[3], f2[2], f3[2], f2[1], f3[1], &fm, &fe, &fv, &fu);
+        s->w[20][c] = fm; s->w[21][c] = fu; s->w[22][c] = fv; s->w[23][c] = fe;
+    }
+}
+
+static void kh_fill_prim_ghosts(ora_sim* s) {
+    for (int y = -2; y < s->ny + 2; y++)
+        for (int x = -2; x < s->nx + 2; x++) {
+            if (x >= 0 && x < s->nx && y >= 0 && y < s->ny) continue;
+            int mx, my;
+            double sg;
+            kh_map(s, x, y, &mx, &my, &sg);
+            long c = pidx(s, x, y), m = pidx(s, mx, my);
+            s->w[0][c] = s->w[0][m];
+            s->w[1][c] = s->w[1][m];
+            s->w[2][c] = sg * s->w[2][m];
+            s->w[3][c] = s->w[3][m];
+        }
+}
+
+/* periodic ghosts alias their real cell (stale or not); wall ghosts are
+   computed from the mirrored primitives like any cell */
+static void kh_fill_grad_ghosts(ora_sim* s) {
+    for (int y = -1; y <= s->ny; y++)
+        for (int x = -1; x <= s->nx; x++) {
+            if (x >= 0 && x < s->nx && y >= 0 && y < s->ny) continue;
+            int wall = s->p.bc_y == DBA_BC_WALL && (y < 0 || y >= s->ny);
+            if (wall) {
+                kh_grads_at(s, x, y);
+            } else {
+                int mx = map_i(x, s->nx, s->p.bc_x), my = map_i(y, s->ny, s->p.bc_y);
+                long c = pidx(s, x, y), m = pidx(s, mx, my);
+                for (int k = 4; k <= 15; k++) s->w[k][c] = s->w[k][m];
+                s->w[28][c] = s->w[28][m];
+            }
+        }
+}
+
+static void kh_step(ora_sim* s) {
+    const int dba = s->p.mode != DBA_MODE_REGULAR;
+    /* 1. primitives on awake blocks (getPrimitive_wakeup) */
+    for (int y = 0; y < s->ny; y++)
+        for (int x = 0; x < s->nx; x++) {
+            if (!active(s, 0, x, y, 0)) continue;
+            long i = (long)x + (long)y * s->nx, c = pidx(s, x, y);
+            kh_prims_of(s, i, &s->w[0][c], &s->w[1][c], &s->w[2][c], &s->w[3][c]);
+        }
+    kh_fill_prim_ghosts(s);
+    /* time step: fixed, or CFL over every cell's current primitives */
+    if (s->p.dt_mode == DBA_DT_CFL) {
+        double vmax = 0.0;
+        for (int y = 0; y < s->ny; y++)
+            for (int x = 0; x < s->nx; x++) {
+                long c = pidx(s, x, y);
+                double sp = kh_speed(s, s->w[0][c], s->w[1][c], s->w[2][c], s->w[3][c]);
+                if (sp > vmax) vmax = sp;
+            }
+        s->dt_kh = s->p.cfl * (s->p.dx / vmax);
+    } else {
+        s->dt_kh = s->p.dt;
+    }
+    /* 2. gradients, primes, metric on awake blocks (extrap_in_time_wakeup) */
+    for (int y = 0; y < s->ny; y++)
+        for (int x = 0; x < s->nx; x++)
+            if (active(s, 0, x, y, 0)) kh_grads_at(s, x, y);
+    kh_fill_grad_ghosts(s);
+    /* 3. fluxes on awake blocks (calc_flux_wakeup); walls: the bottom face of row 0 */
+    int wall = s->p.bc_y == DBA_BC_WALL;
+    for (int y = 0; y < s->ny; y++)
+        for (int x = 0; x < s->nx; x++) {
+            if (!active(s, 0, x, y, 0)) continue;
+            kh_face_fluxes(s, x, y, 1, 1);
+            if (wall && y == 0) kh_face_fluxes(s, x, -1, 0, 1);
+        }
+    /* 4. apply fluxes on awake blocks (apply_flux_wakeup) */
+    double dtdx = s->dt_kh * s->p.dx;
+    int cons = s->p.mode == DBA_MODE_CONSERVED;
+    for (int y = 0; y < s->ny; y++)
+        for (int x = 0; x < s->nx; x++) {
+            long i = (long)x + (long)y * s->nx;
+            if (!active(s, 0, x, y, 0)) {
+                if (cons) for (int q = 0; q < 4; q++) s->w[24 + q][i] = 0.0;
+                continue;
+            }
+            int xm = map_i(x - 1, s->nx, s->p.bc_x);
+            long c = pidx(s, x, y), cxm = pidx(s, xm, y);
+            long cym = (wall && y == 0) ? pidx(s, x, -1) : pidx(s, x, map_i(y - 1, s->ny, s->p.bc_y));
+            /* U order Mass, Momx, Momy, Energy; flux arrays 16.. (x) and 20.. (y) in that order */
+            for (int q = 0; q < 4; q++) {
+                double ch = dtdx * (-1.0 * s->w[16 + q][c] + s->w[16 + q][cxm] - 1.0 * s->w[20 + q][c] + s->w[20 + q][cym]);
+                if (cons) s->w[24 + q][i] = ch;
+                s->f[q][i] += ch;
+            }
+        }
+    if (!dba) return;
+    if (cons) {
+        /* example_Kelvin_Helmholtz/Conserved/KH_DBA.cu:1553-1575: every cell shifted */
+        for (int q = 0; q < 4; q++) {
+            double sum = 0.0;
+            for (long i = 0; i < s->n; i++) sum += s->w[24 + q][i];
+            double avg = sum / (double)s->n;
+            for (long i = 0; i < s->n; i++) s->f[q][i] -= avg;
+        }
+    }
+    /* 5. Wakeup_Next: any(metric > Metric_eps), then 3x3 dilation */
+    memset(s->next[0], 0, (size_t)s->nt);
+    for (int y = 0; y < s->ny; y++)
+        for (int x = 0; x < s->nx; x++) {
+            int t = tile_of(s, x, y, 0);
+            if (s->awake[0][t] && s->w[28][pidx(s, x, y)] > s->p.thresh_a) s->next[0][t] = 1;
+        }
+    dilate_set(s, 0);
+}
+
 /* ------------------------------------------------------------------ public */
 
 static void derive_constants(ora_sim* s) {
@@ -354,6 +584,7 @@ static int nfields(int model) {
     switch (model) {
         case DBA_MODEL_CH: return 1;
         case DBA_MODEL_DENDRITE_K93: return 2;
+        case DBA_MODEL_KH: return 4;
         default: return 0;
     }
 }
@@ -376,8 +607,8 @@ ora_sim* ora_create(const dba_params* p) {
         s->f[k] = (double*)calloc((size_t)s->n, sizeof(double));
         s->fn[k] = (double*)calloc((size_t)s->n, sizeof(double));
     }
-    long nframe = (long)(s->nx + 2) * (s->ny + 2);
-    for (int k = 0; k < 8; k++) s->w[k] = (double*)calloc((size_t)(nframe > s->n ? nframe : s->n), sizeof(double));
+    long nframe = (long)(s->nx + 4) * (s->ny + 4);
+    for (int k = 0; k < MAXW; k++) s->w[k] = (double*)calloc((size_t)(nframe > s->n ? nframe : s->n), sizeof(double));
     for (int k = 0; k < 2; k++) {
         s->awake[k] = (uint8_t*)malloc((size_t)s->nt);
         memset(s->awake[k], 1, (size_t)s->nt);  /* all blocks awake at start */
@@ -416,11 +647,12 @@ int ora_step(ora_sim* s, int nsteps) {
         switch (s->p.model) {
             case DBA_MODEL_CH: ch_step(s); break;
             case DBA_MODEL_DENDRITE_K93: k93_step(s); break;
+            case DBA_MODEL_KH: kh_step(s); break;
             default: return -1;
         }
         s->steps++;
-        s->last_dt = s->p.dt;
-        s->time += s->p.dt;
+        s->last_dt = s->p.model == DBA_MODEL_KH ? s->dt_kh : s->p.dt;
+        s->time += s->last_dt;
     }
     return 0;
 }
@@ -494,6 +726,15 @@ static void* slab_worker(void* argp) {
 
 double ora_bench(ora_sim* s, int nsteps, int nthreads) {
     if (s->p.mode != DBA_MODE_REGULAR) return -1.0;
+    if (s->p.model == DBA_MODEL_KH) {
+        /* KH: serial step loop (ghost fills and the CFL maximum are global) */
+        struct timespec a, b;
+        clock_gettime(CLOCK_MONOTONIC, &a);
+        for (int k = 0; k < nsteps; k++) kh_step(s);
+        clock_gettime(CLOCK_MONOTONIC, &b);
+        s->steps += nsteps;
+        return (double)(b.tv_sec - a.tv_sec) + 1e-9 * (double)(b.tv_nsec - a.tv_nsec);
+    }
     if (nthreads < 1) nthreads = 1;
     if (nthreads > s->ny) nthreads = s->ny;
     struct timespec t0, t1;
