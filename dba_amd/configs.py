@@ -196,3 +196,65 @@ def kh_reference_ic(n, dx=0.002):
     vy = 0.01 * np.sin(4.0 * np.pi * x / n)
     pr = 2.5 * np.ones_like(y)
     return kh_conserved(rho, vx, vy, pr, 5.0 / 3.0, dx * dx)
+
+
+def modelb_ic(nx, ny, nz, x0=0.2, y0=0.2, z0=0.01, seed=1234):
+    """Initialize of example_Reactive_ModelB/kernel.cu:209-211 with host uniforms:
+    x = x0 + 0.05 (u - 1/2), y = y0 + 0.05 (u - 1/2), z = z0 + 0.001 (u - 1/2);
+    three independent streams (seeds seed, seed+1, seed+2) of splitmix64."""
+    n = nx * ny * nz
+    shp = (nz, ny, nx)
+    x = x0 + 0.05 * (uniform01(seed, n) - 0.5)
+    y = y0 + 0.05 * (uniform01(seed + 1, n) - 0.5)
+    z = z0 + 0.001 * (uniform01(seed + 2, n) - 0.5)
+    return x.reshape(shp), y.reshape(shp), z.reshape(shp)
+
+
+def modelb_params(n=256, nz=None, tile=4, mode=abi.MODE_DBA, eps=1e-6, dt=0.0005, history_cap=0, **kw):
+    """The reference ternary Model B (example_Reactive_ModelB/Input.txt)."""
+    nz = n if nz is None else nz
+    p = base_params(abi.MODEL_MODELB, n, n, nz, tile, mode)
+    p.tile_z = tile if nz > 1 else 1
+    p.bc_x = p.bc_y = p.bc_z = abi.BC_PERIODIC
+    p.dx = p.dy = p.dz = 1.0
+    p.dt = dt
+    p.chi_xy = p.chi_xz = p.chi_yz = p.chi_xb = p.chi_yb = 0.0
+    p.chi_zb = 4.0
+    p.v_x, p.v_y, p.v_z = 1.0, 1.0, 2.0
+    p.r_x, p.r_y, p.r_z = 1.0, 1.0, 2.0
+    p.n_exp = p.m_exp = 1.0
+    p.epsx_sq = p.epsy_sq = p.epsz_sq = 4.0
+    p.epsphi_sq = 0.01
+    p.mob_x = p.mob_y = p.mob_z = p.mob_phi = 1.0
+    p.k_eq, p.z_crit, p.p_gel, p.k_0 = 100.0, 0.4, 1.0, 1.0
+    p.thresh_a = eps
+    p.history_cap = history_cap
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def dendrite_k94_params(n=256, nz=None, tile=4, mode=abi.MODE_DBA, history_cap=0, **kw):
+    """The reference 3D dendrite (example_Dendritic_growth/Input.txt)."""
+    nz = n if nz is None else nz
+    p = base_params(abi.MODEL_DENDRITE_K94, n, n, nz, tile, mode)
+    p.tile_z = tile if nz > 1 else 1
+    p.bc_x = p.bc_y = p.bc_z = abi.BC_PERIODIC
+    p.dx = p.dy = p.dz = 0.02
+    p.dt = 0.00005
+    p.delta, p.tau, p.eps_bar, p.alpha, p.gamma = 0.5, 0.0003, 0.01, 0.9, 40.0
+    p.thresh_a = 0.01
+    p.thresh_b = 1e-5
+    p.history_cap = history_cap
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def dendrite_k94_ic(nx, ny, nz, Delta=0.25):
+    """Initialize of example_Dendritic_growth/kernel.cu:167-176: p = 1 - 0.2 dist
+    within dist <= 5 of (Nx/2, Ny/2, Nz/2), T = -Delta."""
+    z, y, x = np.mgrid[0:nz, 0:ny, 0:nx]
+    dist = np.sqrt(((x - nx // 2) ** 2 + (y - ny // 2) ** 2 + (z - nz // 2) ** 2).astype(np.float64))
+    p = np.where(dist <= 5, 1.0 - 0.2 * dist, 0.0)
+    return p, np.full((nz, ny, nx), -Delta)

@@ -24,6 +24,7 @@
 #include "k_ch.h"
 #include "k_k93.h"
 #include "k_kh.h"
+#include "k_ref3d.h"
 
 using namespace dba;
 
@@ -60,6 +61,8 @@ int nfields_of(int model) {
         case DBA_MODEL_CH: return 1;
         case DBA_MODEL_DENDRITE_K93: return 2;
         case DBA_MODEL_KH: return 4;
+        case DBA_MODEL_DENDRITE_K94: return 2;
+        case DBA_MODEL_MODELB: return 4;
         default: return 0;
     }
 }
@@ -83,7 +86,7 @@ struct Instance {
 
 struct Group {
     std::vector<int> ids;
-    int model = 0, tile = 0, threads = 0;
+    int model = 0, tile = 0, tile_z = 1, threads = 0;
     Inst* d_insts = nullptr;
     Seg* d_segs = nullptr;
     InstSegs* d_isegs = nullptr;
@@ -161,6 +164,19 @@ dba_status validate(const dba_params& p) {
         if (!(p.gas_gamma > 1.0)) return fail(DBA_ERR_INVALID, "gas_gamma must exceed 1");
         if (p.dy != p.dx) return fail(DBA_ERR_INVALID, "KH needs dy == dx");
     }
+    if (p.model == DBA_MODEL_DENDRITE_K94 || p.model == DBA_MODEL_MODELB) {
+        const int tz = p.tile_z > 0 ? p.tile_z : 1;
+        if (p.tile_x != 4 || p.tile_y != 4 || (tz != 4 && tz != 1) || (tz == 1) != (p.nz == 1))
+            return fail(DBA_ERR_INVALID, "reference 3D models use 4x4x4 blocks (4x4x1 when nz == 1)");
+        if (p.nx % 4 || p.ny % 4 || p.nz % tz)
+            return fail(DBA_ERR_INVALID, "grid %dx%dx%d must be a multiple of the block", p.nx, p.ny, p.nz);
+        if (p.bc_x != DBA_BC_PERIODIC || p.bc_y != DBA_BC_PERIODIC || p.bc_z != DBA_BC_PERIODIC)
+            return fail(DBA_ERR_INVALID, "reference 3D models are periodic");
+        if (p.model == DBA_MODEL_DENDRITE_K94 && p.mode == DBA_MODE_CONSERVED)
+            return fail(DBA_ERR_INVALID, "conserved DBA is defined for Model B and KH only");
+        if (p.model == DBA_MODEL_MODELB && (p.r_x == 0.0 || p.r_y == 0.0 || p.r_z == 0.0))
+            return fail(DBA_ERR_INVALID, "Model B needs nonzero rx, ry, rz");
+    }
     if (p.model == DBA_MODEL_CH && p.stencil != 5 && p.stencil != 9)
         return fail(DBA_ERR_INVALID, "CH stencil must be 5 or 9");
     if (p.model == DBA_MODEL_CH && p.mode == DBA_MODE_CONSERVED && p.bc_x != DBA_BC_PERIODIC)
@@ -196,6 +212,7 @@ Layout layout_of(const dba_params& p) {
     if (p.mode != DBA_MODE_REGULAR) {
         if (p.model == DBA_MODEL_CH) L.naux = 1;   // mu ring
         if (p.model == DBA_MODEL_KH) L.naux = 24;  // edge state of sleeping tiles
+        if (p.model == DBA_MODEL_MODELB) L.naux = 3;  // chemical potentials of block surfaces
     }
     return L;
 }
@@ -308,8 +325,9 @@ dba_status get_group(dba_ctx* ctx, const std::vector<int>& ids, Group** out) {
     Instance* first = ctx->inst[ids[0]];
     g->model = first->p.model;
     g->tile = first->p.tile_x;
+    g->tile_z = first->p.tile_z;
     g->threads = first->p.threads ? first->p.threads : default_threads(g->model, g->tile);
-    int64_t cap = 4096;
+    int64_t cap = (g->model == DBA_MODEL_DENDRITE_K94 || g->model == DBA_MODEL_MODELB) ? 16384 : 4096;
     g->grid = (int)std::max<int64_t>(1, std::min<int64_t>(g->total_tiles, cap));
     size_t n = ids.size();
     HIP_TRY(hipMalloc(&g->d_insts, sizeof(Inst) * n));
@@ -370,6 +388,25 @@ void launch_step_nt(const Group* g, hipStream_t s) {
 }
 
 void launch_step(const Group* g, hipStream_t s) {
+    if (g->model == DBA_MODEL_DENDRITE_K94 || g->model == DBA_MODEL_MODELB) {
+        dim3 grid(g->grid), block(64);
+        if (g->model == DBA_MODEL_MODELB) {
+            if (g->tile_z == 4)
+                hipLaunchKernelGGL((modelb_step_kernel<4, 4>), grid, block, 0, s, g->d_insts, g->d_segs,
+                                   g->d_offsets, g->nseg);
+            else
+                hipLaunchKernelGGL((modelb_step_kernel<4, 1>), grid, block, 0, s, g->d_insts, g->d_segs,
+                                   g->d_offsets, g->nseg);
+        } else {
+            if (g->tile_z == 4)
+                hipLaunchKernelGGL((k94_step_kernel<4, 4>), grid, block, 0, s, g->d_insts, g->d_segs, g->d_offsets,
+                                   g->nseg);
+            else
+                hipLaunchKernelGGL((k94_step_kernel<4, 1>), grid, block, 0, s, g->d_insts, g->d_segs, g->d_offsets,
+                                   g->nseg);
+        }
+        return;
+    }
     switch (g->tile) {
         case 8: launch_step_nt<8>(g, s); break;
         case 16: launch_step_nt<16>(g, s); break;
@@ -410,7 +447,8 @@ dba_status partition(dba_ctx* ctx, const int32_t* ids, int32_t n_ids, std::vecto
         for (auto& part : *parts) {
             const dba_params& q = ctx->inst[part[0]]->p;
             int thq = q.threads ? q.threads : default_threads(q.model, q.tile_x);
-            if (q.model == p.model && q.tile_x == p.tile_x && thq == thr && part.size() < 255) {
+            if (q.model == p.model && q.tile_x == p.tile_x && q.tile_z == p.tile_z && thq == thr &&
+                part.size() < 255) {
                 part.push_back(id);
                 placed = true;
                 break;
@@ -563,7 +601,11 @@ dba_status dba_instance_create(dba_ctx* ctx, const dba_params* pp, int32_t* id) 
     h.nsets = L.nsets;
     h.nfields = L.nf;
     h.ncons = 0;
-    if (p.mode == DBA_MODE_CONSERVED) h.ncons = p.model == DBA_MODEL_CH ? 1 : (p.model == DBA_MODEL_KH ? 4 : 0);
+    if (p.mode == DBA_MODE_CONSERVED) {
+        if (p.model == DBA_MODEL_CH) h.ncons = 1;
+        if (p.model == DBA_MODEL_KH) h.ncons = 4;
+        if (p.model == DBA_MODEL_MODELB) h.ncons = 3;
+    }
     for (int f = 0; f < kMaxFields; f++) h.field_set[f] = 0;
     if (L.nsets == 2) h.field_set[1] = 1;
     h.full_tiles = (p.nx % p.tile_x == 0 && p.ny % p.tile_y == 0 && p.nz % p.tile_z == 0) ? 1 : 0;

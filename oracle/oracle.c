@@ -565,6 +565,185 @@ static void kh_step(ora_sim* s) {
     dilate_set(s, 0);
 }
 
+
+/* ------------------------------------------------ reference 3D models (periodic) */
+
+static inline long idx3(const ora_sim* s, int x, int y, int z) {
+    return (long)map_i(x, s->nx, DBA_BC_PERIODIC) + (long)map_i(y, s->ny, DBA_BC_PERIODIC) * s->nx +
+           (long)map_i(z, s->nz, DBA_BC_PERIODIC) * s->nx * s->ny;
+}
+
+/* Kobayashi-94 dendrite, example_Dendritic_growth/kernel.cu:181-293 (regular)
+   and :322-634 (DBA). Pi = 3.14159 (kernel.cu:7); factors as kernel.cu:98-101.
+   The powers pow(d,2), pow(d,4), pow(v2,2) are evaluated as products.
+   Regular T: T + dt lapT + p_new - p (kernel.cu:289, left to right);
+   DBA T: (T + dt lapT) + (p_new - p) (kernel.cu:471, 487).
+   w[0] = metric_T. */
+static void k94_step(ora_sim* s) {
+    const double f0 = 1. / (2 * s->p.dx), f1 = 1. / s->p.dx / s->p.dx, f2 = -s->p.alpha / 3.14159, f3 = 1. / s->p.tau;
+    const double dt = s->p.dt, eps = s->p.eps_bar, delta = s->p.delta, gam = s->p.gamma;
+    const int dba = s->p.mode != DBA_MODE_REGULAR;
+    const double* p = s->f[0];
+    const double* T = s->f[1];
+    double* pn = s->fn[0];
+    double* Tn = s->fn[1];
+    for (int z = 0; z < s->nz; z++)
+        for (int y = 0; y < s->ny; y++)
+            for (int x = 0; x < s->nx; x++) {
+                long i = (long)x + (long)y * s->nx + (long)z * s->nx * s->ny;
+                int ap = active(s, 0, x, y, z), at = active(s, 1, x, y, z);
+                if (ap) {
+                    double pxp = p[idx3(s, x + 1, y, z)], pxm = p[idx3(s, x - 1, y, z)];
+                    double pyp = p[idx3(s, x, y + 1, z)], pym = p[idx3(s, x, y - 1, z)];
+                    double pzp = p[idx3(s, x, y, z + 1)], pzm = p[idx3(s, x, y, z - 1)];
+                    double dpx = (pxp - pxm) * f0, dpy = (pyp - pym) * f0, dpz = (pzp - pzm) * f0;
+                    double lapP = (pxp + pxm + pyp + pym + pzp + pzm - 6 * p[i]) * f1;
+                    double v2 = dpx * dpx + dpy * dpy + dpz * dpz;
+                    double sigma;
+                    if (v2 <= 1E-6)
+                        sigma = 1;
+                    else
+                        sigma = 1 - delta * (1 - ((dpx * dpx) * (dpx * dpx) + (dpy * dpy) * (dpy * dpy) +
+                                                  (dpz * dpz) * (dpz * dpz)) / (v2 * v2));
+                    double m = f2 * atan(gam * sigma * T[i]);
+                    double p_change = f3 * (eps * eps * lapP + p[i] * (1 - p[i]) * (p[i] - 0.5 + m));
+                    pn[i] = p[i] + dt * p_change;
+                }
+                if (at || (!dba && ap)) {
+                    double lapT = (T[idx3(s, x + 1, y, z)] + T[idx3(s, x - 1, y, z)] + T[idx3(s, x, y + 1, z)] +
+                                   T[idx3(s, x, y - 1, z)] + T[idx3(s, x, y, z + 1)] + T[idx3(s, x, y, z - 1)] -
+                                   6 * T[i]) * f1;
+                    if (!dba) {
+                        Tn[i] = T[i] + dt * lapT + pn[i] - p[i];
+                    } else {
+                        double tn = T[i] + dt * lapT;
+                        if (ap) tn += pn[i] - p[i];
+                        Tn[i] = tn;
+                        s->w[0][i] = fabs(tn - T[i]);
+                    }
+                }
+            }
+    if (!dba) {
+        memcpy(s->f[0], s->fn[0], sizeof(double) * s->n);
+        memcpy(s->f[1], s->fn[1], sizeof(double) * s->n);
+        return;
+    }
+    memset(s->next[0], 0, (size_t)s->nt);
+    memset(s->next[1], 0, (size_t)s->nt);
+    double mp = s->p.thresh_a, mT = s->p.thresh_b;
+    for (int z = 0; z < s->nz; z++)
+        for (int y = 0; y < s->ny; y++)
+            for (int x = 0; x < s->nx; x++) {
+                long i = (long)x + (long)y * s->nx + (long)z * s->nx * s->ny;
+                int t = tile_of(s, x, y, z);
+                if (s->awake[0][t]) {
+                    s->f[0][i] = pn[i];
+                    if (pn[i] >= mp && pn[i] <= (1.0 - mp)) s->next[0][t] = 1;
+                }
+                if (s->awake[1][t]) {
+                    s->f[1][i] = Tn[i];
+                    if (s->w[0][i] >= mT) s->next[1][t] = 1;
+                }
+            }
+    dilate_set(s, 0);
+    dilate_set(s, 1);
+}
+
+/* 27-point isotropic Laplacian, example_Reactive_ModelB/kernel.cu:290-347, same summation order */
+static double lap27(const ora_sim* s, const double* c, int x, int y, int z) {
+#define C3(a, b, d) c[idx3(s, x + (a), y + (b), z + (d))]
+    return 1.0 / (s->p.dx * s->p.dx) *
+           (-64.0 / 15.0 * C3(0, 0, 0) +
+            7.0 / 15.0 * (C3(1, 0, 0) + C3(-1, 0, 0) + C3(0, 1, 0) + C3(0, -1, 0) + C3(0, 0, 1) + C3(0, 0, -1)) +
+            0.1 * (C3(1, 1, 0) + C3(1, -1, 0) + C3(-1, 1, 0) + C3(-1, -1, 0) + C3(1, 0, 1) + C3(0, 1, 1) +
+                   C3(0, -1, 1) + C3(-1, 0, 1) + C3(1, 0, -1) + C3(0, 1, -1) + C3(0, -1, -1) + C3(-1, 0, -1)) +
+            1.0 / 30.0 * (C3(1, 1, 1) + C3(-1, 1, 1) + C3(-1, -1, 1) + C3(1, -1, 1) + C3(1, 1, -1) + C3(-1, 1, -1) +
+                          C3(-1, -1, -1) + C3(1, -1, -1)));
+#undef C3
+}
+
+/* Ternary Flory-Huggins Model B with reaction, example_Reactive_ModelB/kernel.cu:349-437
+   (regular), :469-763 (DBA), example_Reactive_ModelB/Conserved/kernel.cu:526-553, 630-702.
+   f[0..3] = x, y, z, phi; w[0..2] = dfdx, dfdy, dfdz; w[3] = metric |lap z|;
+   w[4..6] = change (conserved); w[7] = b. */
+static void mb_step(ora_sim* s) {
+    const dba_params* P = &s->p;
+    const int dba = P->mode != DBA_MODE_REGULAR, cons = P->mode == DBA_MODE_CONSERVED;
+    double *X = s->f[0], *Y = s->f[1], *Z = s->f[2], *PH = s->f[3];
+    double *dfx = s->w[0], *dfy = s->w[1], *dfz = s->w[2], *B = s->w[7];
+    for (int z = 0; z < s->nz; z++)
+        for (int y = 0; y < s->ny; y++)
+            for (int x = 0; x < s->nx; x++) {
+                if (!active(s, 0, x, y, z)) continue;
+                long i = (long)x + (long)y * s->nx + (long)z * s->nx * s->ny;
+                B[i] = 1.0 - X[i] - Y[i] - Z[i];
+                dfx[i] = -1.0 + 1.0 / P->r_x + P->chi_xy * Y[i] + P->chi_xz * Z[i] + P->chi_xb * B[i] -
+                         P->chi_xb * X[i] - P->chi_yb * Y[i] - P->chi_zb * Z[i] + log(X[i]) / P->r_x - log(B[i]) -
+                         P->epsx_sq * lap27(s, X, x, y, z);
+                dfy[i] = -1.0 + 1.0 / P->r_y + P->chi_xy * X[i] + P->chi_yz * Z[i] + P->chi_yb * B[i] -
+                         P->chi_xb * X[i] - P->chi_yb * Y[i] - P->chi_zb * Z[i] + log(Y[i]) / P->r_y - log(B[i]) -
+                         P->epsy_sq * lap27(s, Y, x, y, z);
+                double lap_z = lap27(s, Z, x, y, z);
+                dfz[i] = -1.0 + 1.0 / P->r_z + P->chi_xz * X[i] + P->chi_yz * Y[i] + P->chi_zb * B[i] -
+                         P->chi_xb * X[i] - P->chi_yb * Y[i] - P->chi_zb * Z[i] + log(Z[i]) / P->r_z - log(B[i]) -
+                         P->p_gel * PH[i] * PH[i] / 2.0 / (1.0 - P->z_crit) - P->epsz_sq * lap_z -
+                         log(P->k_eq) / P->r_z;
+                s->w[3][i] = fabs(lap_z);
+            }
+    for (int z = 0; z < s->nz; z++)
+        for (int y = 0; y < s->ny; y++)
+            for (int x = 0; x < s->nx; x++) {
+                long i = (long)x + (long)y * s->nx + (long)z * s->nx * s->ny;
+                if (!active(s, 0, x, y, z)) {
+                    if (cons) s->w[4][i] = s->w[5][i] = s->w[6][i] = 0.0;
+                    continue;
+                }
+                double R = P->k_0 * (exp(P->n_exp * P->v_x * dfx[i] + P->m_exp * P->v_y * dfy[i]) -
+                                     exp(P->v_z * dfz[i]));
+                double cx = P->dt * P->v_x * (P->mob_x * lap27(s, dfx, x, y, z) - P->n_exp * R);
+                double cy = P->dt * P->v_y * (P->mob_y * lap27(s, dfy, x, y, z) - P->m_exp * R);
+                double cz = P->dt * P->v_z * (P->mob_z * lap27(s, dfz, x, y, z) + R);
+                s->fn[0][i] = X[i] + cx;
+                s->fn[1][i] = Y[i] + cy;
+                s->fn[2][i] = Z[i] + cz;
+                if (cons) {
+                    s->w[4][i] = cx;
+                    s->w[5][i] = cy;
+                    s->w[6][i] = cz;
+                }
+            }
+    if (!dba) {
+        for (int k = 0; k < 3; k++) memcpy(s->f[k], s->fn[k], sizeof(double) * s->n);
+        return;
+    }
+    if (cons) {
+        for (int k = 0; k < 3; k++) {
+            double sum = 0.0;
+            for (long i = 0; i < s->n; i++) sum += s->w[4 + k][i];
+            double avg = sum / (double)s->n;
+            for (long i = 0; i < s->n; i++) s->fn[k][i] -= avg;
+        }
+        for (int k = 0; k < 3; k++) memcpy(s->f[k], s->fn[k], sizeof(double) * s->n);
+    } else {
+        for (int z = 0; z < s->nz; z++)
+            for (int y = 0; y < s->ny; y++)
+                for (int x = 0; x < s->nx; x++)
+                    if (active(s, 0, x, y, z)) {
+                        long i = (long)x + (long)y * s->nx + (long)z * s->nx * s->ny;
+                        for (int k = 0; k < 3; k++) s->f[k][i] = s->fn[k][i];
+                    }
+    }
+    memset(s->next[0], 0, (size_t)s->nt);
+    for (int z = 0; z < s->nz; z++)
+        for (int y = 0; y < s->ny; y++)
+            for (int x = 0; x < s->nx; x++) {
+                int t = tile_of(s, x, y, z);
+                if (s->awake[0][t] && s->w[3][(long)x + (long)y * s->nx + (long)z * s->nx * s->ny] > P->thresh_a)
+                    s->next[0][t] = 1;
+            }
+    dilate_set(s, 0);
+}
+
 /* ------------------------------------------------------------------ public */
 
 static void derive_constants(ora_sim* s) {
@@ -585,6 +764,8 @@ static int nfields(int model) {
         case DBA_MODEL_CH: return 1;
         case DBA_MODEL_DENDRITE_K93: return 2;
         case DBA_MODEL_KH: return 4;
+        case DBA_MODEL_DENDRITE_K94: return 2;
+        case DBA_MODEL_MODELB: return 4;
         default: return 0;
     }
 }
@@ -592,7 +773,8 @@ static int nfields(int model) {
 ora_sim* ora_create(const dba_params* p) {
     int nf = nfields(p->model);
     if (nf == 0) return NULL;
-    if (p->nx <= 0 || p->ny <= 0 || p->nz != 1) return NULL;
+    if (p->nx <= 0 || p->ny <= 0 || p->nz <= 0) return NULL;
+    if (p->nz != 1 && p->model != DBA_MODEL_DENDRITE_K94 && p->model != DBA_MODEL_MODELB) return NULL;
     ora_sim* s = (ora_sim*)calloc(1, sizeof(ora_sim));
     s->p = *p;
     s->nx = p->nx; s->ny = p->ny; s->nz = p->nz;
@@ -648,6 +830,8 @@ int ora_step(ora_sim* s, int nsteps) {
             case DBA_MODEL_CH: ch_step(s); break;
             case DBA_MODEL_DENDRITE_K93: k93_step(s); break;
             case DBA_MODEL_KH: kh_step(s); break;
+            case DBA_MODEL_DENDRITE_K94: k94_step(s); break;
+            case DBA_MODEL_MODELB: mb_step(s); break;
             default: return -1;
         }
         s->steps++;

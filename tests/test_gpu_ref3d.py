@@ -1,0 +1,90 @@
+"""GPU parity of the reference's own 3D models (libdba_hip.so vs the oracle).
+
+Bar: |diff| <= 1e-10 (dendrite: absolute on p and T; Model B: absolute on the
+volume fractions x, y, z of order 0.2), since these steps call atan (dendrite),
+log and exp (Model B), which differ by a few ulp between the device library
+and glibc; identical awake sets; conserved totals to 1e-11.
+"""
+import numpy as np
+import pytest
+
+from dba_amd import abi, configs
+from oracle.oracle import OracleSim
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def pair(ctx, p, fields):
+    inst = ctx.create(p)
+    o = OracleSim(p)
+    for f, a in enumerate(fields):
+        inst.upload(f, a)
+        o.set_field(f, a)
+    return inst, o
+
+
+@pytest.mark.parametrize("mode", [abi.MODE_REGULAR, abi.MODE_DBA, abi.MODE_CONSERVED])
+@pytest.mark.parametrize("nz", [8, 1])
+def test_modelb_matches_oracle(ctx, mode, nz):
+    n = 32
+    X = list(configs.modelb_ic(n, n, nz)) + [np.zeros((nz, n, n))]
+    if mode != abi.MODE_REGULAR:
+        X[0][..., n // 2:] = 0.2
+        X[1][..., n // 2:] = 0.2
+        X[2][..., n // 2:] = 0.01
+    p = configs.modelb_params(n=n, nz=nz, mode=mode, eps=1e-4, history_cap=64)
+    inst, o = pair(ctx, p, X)
+    for chunk in (1, 4, 15):
+        inst.step(chunk)
+        o.step(chunk)
+        for f in range(3):
+            d = np.max(np.abs(inst.download(f) - o.get_field(f)))
+            assert d <= TOL, (f, d)
+        if mode != abi.MODE_REGULAR:
+            assert np.array_equal(inst.active(0)[1], o.awake(0))
+    if mode != abi.MODE_REGULAR:
+        assert inst.active(0)[0] < inst.info().ntiles
+    if mode == abi.MODE_CONSERVED:
+        for f in range(3):
+            assert abs(inst.download(f).sum() - X[f].sum()) < 1e-11
+    inst.destroy()
+
+
+@pytest.mark.parametrize("mode", [abi.MODE_REGULAR, abi.MODE_DBA])
+@pytest.mark.parametrize("nz", [32, 1])
+def test_k94_matches_oracle(ctx, mode, nz):
+    n = 32
+    p0, T0 = configs.dendrite_k94_ic(n, n, nz)
+    p = configs.dendrite_k94_params(n=n, nz=nz, mode=mode, history_cap=64)
+    inst, o = pair(ctx, p, [p0, T0])
+    for chunk in (1, 5, 30):
+        inst.step(chunk)
+        o.step(chunk)
+        for f in (0, 1):
+            d = np.max(np.abs(inst.download(f) - o.get_field(f)))
+            assert d <= TOL, (f, d)
+        if mode == abi.MODE_DBA:
+            for s in (0, 1):
+                assert np.array_equal(inst.active(s)[1], o.awake(s))
+    if mode == abi.MODE_DBA:
+        h = inst.history()
+        assert np.array_equal(h, o.history())
+        assert h[-1, 0] < inst.info().ntiles
+    inst.destroy()
+
+
+def test_modelb_reference_size_runs(ctx):
+    """64^3 blocks of the reference Input.txt layout, 50 steps through a graph."""
+    n = 64
+    X = list(configs.modelb_ic(n, n, n)) + [np.zeros((n, n, n))]
+    p = configs.modelb_params(n=n, mode=abi.MODE_DBA)
+    inst = ctx.create(p)
+    for f, a in enumerate(X):
+        inst.upload(f, a)
+    inst.step(50, graph_chunk=25)
+    assert inst.check_finite()
+    b = 1.0 - inst.download(0) - inst.download(1) - inst.download(2)
+    assert b.min() > 0.0
+    inst.destroy()
