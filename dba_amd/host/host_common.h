@@ -1,0 +1,224 @@
+// host_common.h - shared pieces of the drop-in drivers (3D_dendritic_growth,
+// CH, KH_DBA): Input.txt parsing, the reference output layouts, exit codes.
+// The drivers reach the GPU only through include/dba/dba.h.
+#pragma once
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/dba/dba.h"
+
+namespace host {
+
+// Map a library status to the reference's exit codes: CUDACHECK exits 1,
+// CheckLastCudaError exits 2 (example_Dendritic_growth/kernel.cu:12-29).
+inline void check(dba_status st, const char* what) {
+    if (st == DBA_OK) return;
+    std::cout << "DBA error in " << what << ": " << dba_last_error() << std::endl;
+    std::exit(st == DBA_ERR_LAUNCH ? 2 : 1);
+}
+
+// Positional reader with the reference's semantics: each group is read with
+// operator>> and the rest of the line is discarded with getline
+// (example_Dendritic_growth/kernel.cu:82-96). Lines after the last group may
+// carry "key value" extensions, which the reference parsers never read.
+class InputReader {
+  public:
+    explicit InputReader(const std::string& path) : in_(path) {
+        if (!in_.is_open()) {
+            std::cout << "!!!!!Can not open" << path << "!! Exit!!!!!!!!" << std::endl;
+            std::exit(1);
+        }
+        std::cout << "Reading input from " << path << std::endl;
+    }
+    template <typename... T>
+    void group(T&... v) {
+        (void)std::initializer_list<int>{(in_ >> v, 0)...};
+        std::string rest;
+        std::getline(in_, rest);
+    }
+    // "key value" pairs from the remaining lines ('#' starts a comment)
+    std::map<std::string, std::string> extras() {
+        std::map<std::string, std::string> kv;
+        std::string line;
+        while (std::getline(in_, line)) {
+            auto h = line.find('#');
+            if (h != std::string::npos) line = line.substr(0, h);
+            std::istringstream ss(line);
+            std::string k, v;
+            if (ss >> k >> v) kv[k] = v;
+        }
+        return kv;
+    }
+
+  private:
+    std::ifstream in_;
+};
+
+inline double getd(const std::map<std::string, std::string>& kv, const std::string& k, double dflt) {
+    auto it = kv.find(k);
+    return it == kv.end() ? dflt : std::atof(it->second.c_str());
+}
+inline int geti(const std::map<std::string, std::string>& kv, const std::string& k, int dflt) {
+    auto it = kv.find(k);
+    return it == kv.end() ? dflt : std::atoi(it->second.c_str());
+}
+inline std::string gets(const std::map<std::string, std::string>& kv, const std::string& k, const std::string& d) {
+    auto it = kv.find(k);
+    return it == kv.end() ? d : it->second;
+}
+
+// splitmix64 uniforms in [0, 1) (same stream as dba_amd.configs.uniform01)
+inline std::vector<double> uniform01(uint64_t seed, size_t n) {
+    std::vector<double> u(n);
+    for (size_t i = 0; i < n; i++) {
+        uint64_t z = seed + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z = z ^ (z >> 31);
+        u[i] = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    }
+    return u;
+}
+
+struct Grid {
+    int nx, ny, nz;
+    size_t n() const { return (size_t)nx * ny * nz; }
+};
+
+// RECTILINEAR_GRID with named scalars (example_Reactive_ModelB/kernel.cu:766-879,
+// example_Dendritic_growth/kernel.cu:637-707), optional 2-component vectors
+// written as "vx vy 0.0" per line (example_Kelvin_Helmholtz/KH_DBA.cu:1132-1202).
+inline void write_rectilinear(const std::string& name, const Grid& g,
+                              const std::vector<std::pair<std::string, const double*>>& scalars,
+                              const double* vx = nullptr, const double* vy = nullptr) {
+    std::ofstream o(name);
+    o << "# vtk DataFile Version 2.0" << std::endl;
+    o << "OUTPUT by Roy Zhang\n";
+    o << "ASCII" << std::endl;
+    o << "DATASET RECTILINEAR_GRID" << std::endl;
+    o << "DIMENSIONS " << g.nx << " " << g.ny << " " << g.nz << std::endl;
+    o << "X_COORDINATES " << g.nx << " int" << std::endl;
+    for (int i = 0; i < g.nx; i++) o << i << "\t";
+    o << std::endl;
+    o << "Y_COORDINATES " << g.ny << " int" << std::endl;
+    for (int i = 0; i < g.ny; i++) o << i << "\t";
+    o << std::endl;
+    o << "Z_COORDINATES " << g.nz << " int" << std::endl;
+    for (int i = 0; i < g.nz; i++) o << i << "\t";
+    o << std::endl;
+    o << "POINT_DATA " << g.n() << std::endl;
+    bool first = true;
+    for (auto& s : scalars) {
+        if (!first) o << std::endl;
+        first = false;
+        o << "SCALARS " << s.first << " double" << std::endl;
+        o << "LOOKUP_TABLE default" << std::endl;
+        for (size_t i = 0; i < g.n(); i++) o << s.second[i] << " ";
+    }
+    o << std::endl;
+    if (vx && vy) {
+        o << "VECTORS vec double" << std::endl;
+        for (size_t i = 0; i < g.n(); i++) o << vx[i] << " " << vy[i] << " 0.0 " << std::endl;
+    }
+}
+
+// STRUCTURED_POINTS with one scalar named after the prefix: the layout
+// Compare_VTK and Extract_Tip parse (example_Dendritic_growth/CPU_version/Base.cpp:230-262).
+inline void write_structured(const std::string& prefix, int t, const Grid& g, const double* data) {
+    std::ofstream o(prefix + "_" + std::to_string(t) + ".vtk");
+    o << "# vtk DataFile Version 2.0" << std::endl;
+    o << prefix << std::endl << "ASCII" << std::endl << "DATASET STRUCTURED_POINTS" << std::endl;
+    o << "DIMENSIONS " << g.nx << " " << g.ny << " " << g.nz << std::endl;
+    o << "ASPECT_RATIO 1 1 1" << std::endl << "ORIGIN 0 0 0" << std::endl << "POINT_DATA " << g.n() << std::endl;
+    o << "SCALARS " << prefix << " double" << std::endl << "LOOKUP_TABLE default" << std::endl;
+    for (size_t i = 0; i < g.n(); i++) o << data[i] << " ";
+    o << std::endl;
+}
+
+// Per-point awake flags of the blocks (Output_wakeup, example_Reactive_ModelB/kernel.cu:882-917).
+inline void write_wakeup(const std::string& prefix, int t, const Grid& g, int bx, int by, int bz, int nbx, int nby,
+                         const std::vector<std::pair<std::string, const uint8_t*>>& sets) {
+    std::ofstream o(prefix + "_" + std::to_string(t) + ".vtk");
+    o << "# vtk DataFile Version 2.0" << std::endl;
+    o << "wakeup" << std::endl << "ASCII" << std::endl << "DATASET STRUCTURED_POINTS" << std::endl;
+    o << "DIMENSIONS " << g.nx << " " << g.ny << " " << g.nz << std::endl;
+    o << "ASPECT_RATIO 1 1 1" << std::endl << "ORIGIN 0 0 0" << std::endl << "POINT_DATA " << g.n() << std::endl;
+    bool first = true;
+    for (auto& s : sets) {
+        if (!first) o << std::endl;
+        first = false;
+        o << "SCALARS " << s.first << " int" << std::endl << "LOOKUP_TABLE default" << std::endl;
+        for (int z = 0; z < g.nz; z++)
+            for (int y = 0; y < g.ny; y++)
+                for (int x = 0; x < g.nx; x++) o << (int)s.second[x / bx + (y / by) * nbx + (z / bz) * nbx * nby] << " ";
+    }
+    o << std::endl;
+}
+
+// Appends a line; writes the header first when the file is new
+// (Calc_RealTime_Values, example_Dendritic_growth/kernel.cu:797-817).
+inline void append_line(const std::string& name, const std::string& header, const std::string& line) {
+    std::ifstream probe(name);
+    bool exists = probe.good();
+    probe.close();
+    std::ofstream o(name, std::fstream::app);
+    if (!o.is_open()) {
+        std::cout << "!!!!!Can not open" << name << "!! Exit!!!!!!!!" << std::endl;
+        std::exit(1);
+    }
+    if (!exists && !header.empty()) o << header << std::endl;
+    o << line << std::endl;
+}
+
+struct Clock {
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    double seconds() const {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    void reset() { t0 = std::chrono::steady_clock::now(); }
+};
+
+// One context + instance with the library, field download helpers.
+struct Sim {
+    dba_ctx* ctx = nullptr;
+    int32_t id = -1;
+    dba_params p{};
+    Grid g{};
+    int nt = 0;
+    explicit Sim(const dba_params& prm) : p(prm), g{prm.nx, prm.ny, prm.nz} {
+        check(dba_init(0, &ctx), "dba_init");
+        check(dba_instance_create(ctx, &p, &id), "dba_instance_create");
+        dba_info info;
+        check(dba_instance_info(ctx, id, &info), "dba_instance_info");
+        nt = info.ntiles;
+    }
+    ~Sim() {
+        if (ctx) dba_finalize(ctx);
+    }
+    void upload(int f, const std::vector<double>& a) { check(dba_upload(ctx, id, f, a.data(), a.size()), "upload"); }
+    std::vector<double> download(int f) {
+        std::vector<double> a(g.n());
+        check(dba_download(ctx, id, f, a.data(), a.size()), "download");
+        return a;
+    }
+    void step(int n) {
+        if (n > 0) check(dba_step_graph(ctx, &id, 1, n, std::min(n, 100)), "dba_step");
+        check(dba_sync(ctx), "dba_sync");
+    }
+    std::vector<uint8_t> awake(int set, int* count) {
+        std::vector<uint8_t> m((size_t)nt);
+        check(dba_active_tiles(ctx, id, set, count, m.data()), "dba_active_tiles");
+        return m;
+    }
+};
+
+}  // namespace host
