@@ -9,6 +9,17 @@ Workloads (--workload):
   dendrite  2D Kobayashi dendrite 512^2 FP64, 16x16 tiles, DBA.
   kh        compressible Euler Kelvin-Helmholtz 2048^2 FP64, gamma 1.4, CFL 0.4,
             reflective y walls, 16x16 tiles, density-gradient wake-up (eps 0.1).
+  batch     64 independent 2D Kobayashi dendrites, sides log-uniform in [256, 2048]
+            (rounded to multiples of 32), per-instance delta, j, K and 1-3 seeds
+            (dba_amd.batch), 5000 steps, one merged-worklist group per step;
+            tile and workgroup size from the autotuner. Reports the batched
+            rate next to the plain one-instance-at-a-time loop (16x16 tiles).
+  serve     Poisson stream of dendrite and reactive-CH solves (dba_amd.serve):
+            sides log-uniform in [256, 4096], steps log-uniform in [1e3, 5e4],
+            admitted by device footprint against a memory budget (--budget-gib),
+            arrival rate steered toward 80% budget occupancy; reports p50/p99
+            completion latency, the aggregate rate, and the plain sequential
+            rate of a sample of the same requests run alone.
 A step is one DBA time step of the whole instance (stencil kernel + finalize
 kernel), replayed from a captured hipGraph. Inputs are resident in HBM before
 the timed region. value = active cell-updates/s over the timed steps (device
@@ -37,6 +48,8 @@ WORKLOADS = {
     "ch_cons": dict(model="ch", n=1024, tile=32, mode="conserved", b_alg=16, warm=500, steps=5000),
     "dendrite": dict(model="k93", n=512, tile=16, mode="dba", b_alg=32, warm=200, steps=4000),
     "kh": dict(model="kh", n=2048, tile=16, mode="dba", b_alg=64, warm=100, steps=1000),
+    "batch": dict(model="k93", b_alg=32, warm=0, steps=5000),
+    "serve": dict(model="mix", b_alg=None, warm=0, steps=0),
 }
 
 
@@ -76,21 +89,160 @@ def cpu_baseline(workload, budget_s=12.0):
         t = o.bench(steps, nth)
         res[label] = (ncell * steps / t, steps, nth)
         o.close()
-    cpu_model = ""
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                cpu_model = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
     v, steps, nth = res["threads"]
     return {
         "value": v, "unit": "cell-updates/s", "cores": nth, "kind": "port",
-        "sample": f"{steps} regular-mode steps of the {p.nx}x{p.ny} grid, {nth} y-slab threads ({cpu_model})",
+        "sample": f"{steps} regular-mode steps of the {p.nx}x{p.ny} grid, {nth} y-slab threads ({cpu_model()})",
         "serial_value": res["serial"][0],
         "serial_sample": f"{res['serial'][1]} regular-mode steps, 1 thread",
     }
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return ""
+
+
+def kernel_roofline(ctx, insts, b_alg, steps):
+    """Eager pass with hipEvents around every step kernel: average duration and
+    algorithmic GB/s (b_alg x active cells of the launch / duration)."""
+    ctx.timing(True)
+    u0 = sum(i.info().cell_updates for i in insts)
+    ctx.step(insts, steps)
+    ctx.sync()
+    u1 = sum(i.info().cell_updates for i in insts)
+    k_ms, k_n, f_ms, f_n = ctx.timing_query()
+    ctx.timing(False)
+    avg_k_s = k_ms / 1e3 / max(1, k_n)
+    achieved = b_alg * (u1 - u0) / max(1, k_n) / avg_k_s / 1e9
+    return {
+        "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        "kernel": "step kernel (fused stencil + detector)", "b_alg_per_cell": b_alg,
+        "avg_kernel_us": avg_k_s * 1e6, "avg_finalize_us": f_ms * 1e3 / max(1, f_n), "launches_timed": k_n,
+    }
+
+
+def bench_batch(args, K):
+    """64-instance dendrite sweep: autotuned batched run vs the plain loop."""
+    from dba_amd import Context, batch, configs
+    from oracle.oracle import OracleSim
+    items = batch.make_sweep(64)
+    t0 = time.perf_counter()
+    cache = {it.index: it.fields() for it in items}
+    ic_s = time.perf_counter() - t0
+    log = (lambda m: print(m, file=sys.stderr, flush=True))
+    with Context(0) as ctx:
+        (tile, nt), table = batch.autotune(ctx, items, trial_steps=args.tune_steps, fields_cache=cache, log=log)
+        log(f"autotune: tile {tile}, threads {nt}")
+        st, insts = batch.run_batched(ctx, items, K, tile=tile, threads=nt, chunk=args.chunk, fields_cache=cache,
+                                      keep=True)
+        log(f"batched: {st.wall_s:.3f} s, {st.active_rate:.4g} active/s")
+        roof = kernel_roofline(ctx, insts, 32, args.timing_steps)
+        finite = all(i.check_finite() for i in insts)
+        for i in insts:
+            i.destroy()
+        st16, _ = batch.run_batched(ctx, items, K, tile=16, threads=0, chunk=args.chunk, fields_cache=cache)
+        log(f"batched 16x16: {st16.wall_s:.3f} s")
+        seq, _ = batch.run_sequential(ctx, items, K, tile=16, threads=0, chunk=args.chunk, fields_cache=cache)
+        log(f"sequential: {seq.wall_s:.3f} s, {seq.active_rate:.4g} active/s")
+    out = {
+        "metric": "active cell-updates/s of the batched 64-instance dendrite sweep (+ vs plain sequential DBA)",
+        "value": st.active_rate, "unit": "cell-updates/s", "n_gpus": 1, "steps": K, "warmup": 0,
+        "ms_per_step": st.wall_s * 1e3 / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "batch: 64 Kobayashi-93 dendrites, sides log-U[256,2048], 5000 steps",
+                   "tile": tile, "threads": nt, "graph_chunk": args.chunk, **batch.summary(items)},
+        "full_grid_updates_per_s": st.full_rate,
+        "active_fraction": st.active_updates / st.full_updates,
+        "batched_vs_sequential": st.active_rate / seq.active_rate,
+        "sequential": {"value": seq.active_rate, "wall_s": seq.wall_s, "tile": 16, "threads": "default",
+                       "note": "same library, one instance at a time, graph-replayed, stepping time only"},
+        "batched_16x16": {"value": st16.active_rate, "wall_s": st16.wall_s,
+                          "vs_sequential_wall": seq.wall_s / st16.wall_s},
+        "autotune": [{"tile": a, "threads": b, "ms_per_step": c, "active_rate": d} for a, b, c, d in table],
+        "ic_host_s": ic_s, "finite": bool(finite), "roofline": roof,
+    }
+    if not args.no_cpu:
+        # the oracle's regular-mode loop on one 512^2 dendrite of the sweep's parameters
+        it = min(items, key=lambda x: abs(x.side - 512))
+        p = it.params(tile=16)
+        p.mode = 0
+        o = OracleSim(p)
+        phi, T = cache[it.index]
+        o.set_field(0, phi)
+        o.set_field(1, T)
+        cores = max(1, min(len(os.sched_getaffinity(0)), 16))
+        tc = o.bench(2, cores)
+        steps = max(2, min(2000, int(10.0 / max(tc / 2, 1e-6))))
+        tc = o.bench(steps, cores)
+        o.close()
+        out["cpu_baseline"] = {"value": it.cells * steps / tc, "unit": "cell-updates/s", "cores": cores,
+                               "kind": "port", "sample": f"{steps} regular-mode steps of instance {it.index} "
+                               f"({it.side}^2), {cores} y-slab threads ({cpu_model()})"}
+    return out
+
+
+def bench_serve(args):
+    """Continuous-batching server on a Poisson stream; sequential sample alone."""
+    from dba_amd import Context, serve
+    log = (lambda m: print(m, file=sys.stderr, flush=True))
+    rs = serve.make_requests(args.requests)
+    tiles = {"dendrite": 16, "ch": 32}
+    budget = int(args.budget_gib * 2 ** 30)
+    with Context(0) as ctx:
+        free, total = ctx.device_memory()
+        be = serve.LibraryBackend(ctx)
+        srv = serve.Server(be, budget, chunk=args.chunk, tiles=tiles, target_occupancy=0.8, gain=2.0)
+        done = srv.run(rs, rate=args.rate)
+        rep = serve.report(done, srv.wall, srv.occupancy)
+        log(f"served {rep['completed']} in {srv.wall:.1f} s, p50 {rep['p50_latency_s']:.3f} s, "
+            f"p99 {rep['p99_latency_s']:.3f} s, {rep['active_rate']:.4g} active/s")
+        # plain sequential loop: requests alone, in stream order, until the time budget
+        seq_t = seq_u = 0.0
+        nseq = 0
+        for r in rs:
+            if seq_t > args.seq_budget_s:
+                break
+            inst = ctx.create(r.params(tiles, {}))
+            for f, a in enumerate(r.fields()):
+                inst.upload(f, a)
+            ctx.sync()
+            u0 = inst.info().cell_updates
+            t0 = time.perf_counter()
+            inst.step(r.steps, graph_chunk=args.chunk)
+            ctx.sync()
+            seq_t += time.perf_counter() - t0
+            seq_u += inst.info().cell_updates - u0
+            nseq += 1
+            inst.destroy()
+        seq_rate = seq_u / seq_t
+        log(f"sequential sample: {nseq} requests, {seq_rate:.4g} active/s")
+    lat = sorted(c.latency for c in done)
+    out = {
+        "metric": "aggregate active cell-updates/s of the served dendrite + CH stream (+ p50/p99 latency)",
+        "value": rep["active_rate"], "unit": "cell-updates/s", "n_gpus": 1, "steps": 0, "warmup": 0,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"serve: {args.requests} requests, 50% Kobayashi-93 dendrite / 50% reactive CH, "
+                               "sides log-U[256,4096], steps log-U[1e3,5e4]",
+                   "budget_bytes": budget, "device_total_bytes": total, "graph_chunk": args.chunk,
+                   "initial_rate_per_s": args.rate, "target_occupancy": 0.8},
+        **rep,
+        "latency_s": {"p50": rep["p50_latency_s"], "p90": serve.percentile(lat, 90), "p99": rep["p99_latency_s"],
+                      "max": lat[-1]},
+        "served_vs_sequential": rep["active_rate"] / seq_rate,
+        "sequential": {"value": seq_rate, "requests": nseq, "wall_s": seq_t,
+                       "note": "the first requests of the same stream run alone, one at a time"},
+        "ticks": srv.ticks, "max_used_bytes": srv.max_used,
+        "final_rate_per_s": srv.rate_trace[-1][1] if srv.rate_trace else args.rate,
+    }
+    return out
 
 
 def main():
@@ -102,6 +254,11 @@ def main():
     ap.add_argument("--chunk", type=int, default=100, help="steps per captured hipGraph")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=500)
+    ap.add_argument("--tune-steps", type=int, default=200, help="batch: timed steps per autotuner candidate")
+    ap.add_argument("--requests", type=int, default=300, help="serve: requests in the stream")
+    ap.add_argument("--rate", type=float, default=4.0, help="serve: initial arrival rate (1/s)")
+    ap.add_argument("--budget-gib", type=float, default=4.0, help="serve: device memory budget for admission")
+    ap.add_argument("--seq-budget-s", type=float, default=30.0, help="serve: time for the sequential sample")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported in this build "
@@ -112,6 +269,12 @@ def main():
     w = WORKLOADS[args.workload]
     K = args.steps if args.steps is not None else w["steps"]
     W = args.warmup if args.warmup is not None else w["warm"]
+    if args.workload == "batch":
+        print(json.dumps(bench_batch(args, K)))
+        return
+    if args.workload == "serve":
+        print(json.dumps(bench_serve(args)))
+        return
     p, fields = make(args.workload)
     ncell = p.nx * p.ny
 

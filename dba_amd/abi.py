@@ -41,6 +41,12 @@ DT_CFL = 1
 KH_METRIC_ALLGRAD = 0
 KH_METRIC_DENSITY = 1
 
+# field ids (dba_upload / dba_download)
+F_PHI, F_T = 0, 1                      # dendrite
+F_C = 0                                # CH
+F_X, F_Y, F_Z, F_MBPHI = 0, 1, 2, 3    # Model B
+F_MASS, F_MOMX, F_MOMY, F_ENERGY = 0, 1, 2, 3  # KH
+
 SET_MAIN = 0
 SET_T = 1
 
@@ -106,6 +112,7 @@ SIGNATURES = [
     ("dba_finalize", C.c_int32, [C.c_void_p]),
     ("dba_last_error", C.c_char_p, []),
     ("dba_footprint", C.c_int32, [C.POINTER(Params), C.POINTER(C.c_int64)]),
+    ("dba_device_memory", C.c_int32, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     ("dba_instance_create", C.c_int32, [C.c_void_p, C.POINTER(Params), C.POINTER(C.c_int32)]),
     ("dba_instance_destroy", C.c_int32, [C.c_void_p, C.c_int32]),
     ("dba_instance_info", C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(Info)]),

@@ -81,6 +81,7 @@ struct Instance {
     int64_t footprint = 0;
     int64_t ncell = 0;
     bool needs_prime = false;    // KH: per-tile wave speeds must be recomputed before the next step
+    bool stepped = false;        // work for this instance has been enqueued on the compute stream
     Inst* d_self = nullptr;      // single-instance descriptor (downloads, checks)
 };
 
@@ -97,6 +98,7 @@ struct Group {
     int nseg = 0;
     int64_t total_tiles = 0;
     int grid = 0;
+    uint64_t last_use = 0;
 };
 
 struct GraphKey {
@@ -107,13 +109,20 @@ struct GraphKey {
 
 }  // namespace
 
+struct CachedGraph {
+    hipGraphExec_t exec = nullptr;
+    uint64_t last_use = 0;
+};
+
 struct dba_ctx {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // compute stream: steps, finalize, downloads
+    hipStream_t setup = nullptr;   // instance and group set-up, uploads of never-stepped instances
     int next_id = 1;
+    uint64_t clock = 0;            // LRU clock of the group and graph caches
     std::map<int, Instance*> inst;
     std::map<std::vector<int>, Group*> groups;
-    std::map<GraphKey, hipGraphExec_t> graphs;
+    std::map<GraphKey, CachedGraph> graphs;
     double* staging = nullptr;
     size_t staging_n = 0;
     int* d_flag = nullptr;
@@ -234,12 +243,12 @@ int64_t footprint_of(const dba_params& p) {
 }
 
 template <typename T>
-dba_status dalloc(Instance* in, T** out, size_t count) {
+dba_status dalloc(Instance* in, T** out, size_t count, hipStream_t setup) {
     void* ptr = nullptr;
     size_t bytes = std::max<size_t>(count * sizeof(T), 8);
     hipError_t e = hipMalloc(&ptr, bytes);
     if (e != hipSuccess) return fail(DBA_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
-    e = hipMemset(ptr, 0, bytes);
+    e = hipMemsetAsync(ptr, 0, bytes, setup);
     if (e != hipSuccess) return fail(DBA_ERR_HIP, "hipMemset failed: %s", hipGetErrorString(e));
     in->allocs.push_back(ptr);
     *out = (T*)ptr;
@@ -251,30 +260,80 @@ void free_instance(Instance* in) {
     in->allocs.clear();
 }
 
+void free_group(Group* g) {
+    (void)hipFree(g->d_insts);
+    (void)hipFree(g->d_segs);
+    (void)hipFree(g->d_isegs);
+    (void)hipFree(g->d_parts);
+    (void)hipFree(g->d_counts);
+    (void)hipFree(g->d_offsets);
+    (void)hipFree(g->d_done);
+    delete g;
+}
+
+bool subset(const std::vector<int>& a, const std::vector<int>& b) {  // every id of a is in b
+    for (int x : a)
+        if (std::find(b.begin(), b.end(), x) == b.end()) return false;
+    return true;
+}
+
+// Drops every captured graph that may launch group g (its ids are a subset of
+// the graph's instance set).
+void drop_graphs_using(dba_ctx* ctx, const std::vector<int>& gids) {
+    for (auto it = ctx->graphs.begin(); it != ctx->graphs.end();) {
+        if (subset(gids, it->first.ids)) {
+            (void)hipGraphExecDestroy(it->second.exec);
+            it = ctx->graphs.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+
 void invalidate_groups(dba_ctx* ctx, int id) {
     for (auto it = ctx->groups.begin(); it != ctx->groups.end();) {
         if (std::find(it->first.begin(), it->first.end(), id) != it->first.end()) {
-            Group* g = it->second;
-            (void)hipFree(g->d_insts);
-            (void)hipFree(g->d_segs);
-            (void)hipFree(g->d_isegs);
-            (void)hipFree(g->d_parts);
-            (void)hipFree(g->d_counts);
-            (void)hipFree(g->d_offsets);
-            (void)hipFree(g->d_done);
-            delete g;
+            free_group(it->second);
             it = ctx->groups.erase(it);
         } else {
             ++it;
         }
     }
-    for (auto it = ctx->graphs.begin(); it != ctx->graphs.end();) {
-        if (std::find(it->first.ids.begin(), it->first.ids.end(), id) != it->first.ids.end()) {
-            (void)hipGraphExecDestroy(it->second);
-            it = ctx->graphs.erase(it);
-        } else {
-            ++it;
-        }
+    drop_graphs_using(ctx, {id});
+}
+
+// Bounded caches: a serving loop steps a changing set of instances, and every
+// distinct set has its own groups and graphs. Entries not used by the current
+// call (last_use < clock) are evicted least recently used first.
+constexpr size_t kMaxGroups = 64;
+constexpr size_t kMaxGraphs = 32;
+
+void evict_groups(dba_ctx* ctx) {
+    while (ctx->groups.size() >= kMaxGroups) {
+        auto victim = ctx->groups.end();
+        for (auto it = ctx->groups.begin(); it != ctx->groups.end(); ++it)
+            if (it->second->last_use < ctx->clock &&
+                (victim == ctx->groups.end() || it->second->last_use < victim->second->last_use))
+                victim = it;
+        if (victim == ctx->groups.end()) return;
+        (void)hipStreamSynchronize(ctx->stream);
+        drop_graphs_using(ctx, victim->first);
+        free_group(victim->second);
+        ctx->groups.erase(victim);
+    }
+}
+
+void evict_graphs(dba_ctx* ctx) {
+    while (ctx->graphs.size() >= kMaxGraphs) {
+        auto victim = ctx->graphs.end();
+        for (auto it = ctx->graphs.begin(); it != ctx->graphs.end(); ++it)
+            if (it->second.last_use < ctx->clock &&
+                (victim == ctx->graphs.end() || it->second.last_use < victim->second.last_use))
+                victim = it;
+        if (victim == ctx->graphs.end()) return;
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipGraphExecDestroy(victim->second.exec);
+        ctx->graphs.erase(victim);
     }
 }
 
@@ -302,10 +361,13 @@ dba_status kh_prime(dba_ctx* ctx, Instance* in) {
 dba_status get_group(dba_ctx* ctx, const std::vector<int>& ids, Group** out) {
     auto it = ctx->groups.find(ids);
     if (it != ctx->groups.end()) {
+        it->second->last_use = ctx->clock;
         *out = it->second;
         return DBA_OK;
     }
+    evict_groups(ctx);
     Group* g = new Group();
+    g->last_use = ctx->clock;
     g->ids = ids;
     std::vector<Inst> hs;
     std::vector<Seg> segs;
@@ -331,20 +393,21 @@ dba_status get_group(dba_ctx* ctx, const std::vector<int>& ids, Group** out) {
     g->grid = (int)std::max<int64_t>(1, std::min<int64_t>(g->total_tiles, cap));
     size_t n = ids.size();
     HIP_TRY(hipMalloc(&g->d_insts, sizeof(Inst) * n));
-    HIP_TRY(hipMemcpy(g->d_insts, hs.data(), sizeof(Inst) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync(g->d_insts, hs.data(), sizeof(Inst) * n, hipMemcpyHostToDevice, ctx->setup));
     size_t ns = segs.size();
     HIP_TRY(hipMalloc(&g->d_segs, sizeof(Seg) * ns));
-    HIP_TRY(hipMemcpy(g->d_segs, segs.data(), sizeof(Seg) * ns, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync(g->d_segs, segs.data(), sizeof(Seg) * ns, hipMemcpyHostToDevice, ctx->setup));
     HIP_TRY(hipMalloc(&g->d_isegs, sizeof(InstSegs) * n));
-    HIP_TRY(hipMemcpy(g->d_isegs, isegs.data(), sizeof(InstSegs) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync(g->d_isegs, isegs.data(), sizeof(InstSegs) * n, hipMemcpyHostToDevice, ctx->setup));
     HIP_TRY(hipMalloc(&g->d_parts, sizeof(SegPart) * ns));
-    HIP_TRY(hipMemset(g->d_parts, 0, sizeof(SegPart) * ns));
+    HIP_TRY(hipMemsetAsync(g->d_parts, 0, sizeof(SegPart) * ns, ctx->setup));
     HIP_TRY(hipMalloc(&g->d_counts, sizeof(int32_t) * ns));
     HIP_TRY(hipMalloc(&g->d_offsets, sizeof(int32_t) * (ns + 1)));
     HIP_TRY(hipMalloc(&g->d_done, sizeof(uint32_t)));
-    HIP_TRY(hipMemset(g->d_done, 0, sizeof(uint32_t)));
-    HIP_TRY(hipMemset(g->d_offsets, 0, sizeof(int32_t) * (ns + 1)));
-    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemsetAsync(g->d_done, 0, sizeof(uint32_t), ctx->setup));
+    HIP_TRY(hipMemsetAsync(g->d_offsets, 0, sizeof(int32_t) * (ns + 1), ctx->setup));
+    // the compute stream is used only after this host-side wait
+    HIP_TRY(hipStreamSynchronize(ctx->setup));
     ctx->groups[ids] = g;
     *out = g;
     return DBA_OK;
@@ -488,6 +551,8 @@ dba_status prepare_groups(dba_ctx* ctx, const int32_t* ids, int32_t n_ids, std::
     std::vector<std::vector<int>> parts;
     dba_status st = partition(ctx, ids, n_ids, &parts);
     if (st != DBA_OK) return st;
+    ctx->clock++;
+    for (int k = 0; k < n_ids; k++) ctx->inst[ids[k]]->stepped = true;
     for (auto& part : parts) {
         Group* g = nullptr;
         st = get_group(ctx, part, &g);
@@ -546,6 +611,12 @@ dba_status dba_init(int32_t device, dba_ctx** out) {
         delete ctx;
         return fail(DBA_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
     }
+    e = hipStreamCreateWithFlags(&ctx->setup, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return fail(DBA_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
+    }
     HIP_TRY(hipMalloc(&ctx->d_flag, sizeof(int)));
     for (auto& m : ctx->markers) HIP_TRY(hipEventCreate(&m));
     *out = ctx;
@@ -559,7 +630,7 @@ dba_status dba_finalize(dba_ctx* ctx) {
     std::vector<int> ids;
     for (auto& kv : ctx->inst) ids.push_back(kv.first);
     for (int id : ids) dba_instance_destroy(ctx, id);
-    for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second.exec);
     for (auto& e : ctx->ev_step) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
     for (auto& e : ctx->ev_fin) (void)hipEventDestroy(e.second);
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
@@ -567,6 +638,7 @@ dba_status dba_finalize(dba_ctx* ctx) {
     if (ctx->staging) (void)hipFree(ctx->staging);
     if (ctx->d_flag) (void)hipFree(ctx->d_flag);
     (void)hipStreamDestroy(ctx->stream);
+    (void)hipStreamDestroy(ctx->setup);
     delete ctx;
     return DBA_OK;
 }
@@ -576,6 +648,16 @@ dba_status dba_footprint(const dba_params* p, int64_t* bytes) {
     dba_status st = validate(*p);
     if (st != DBA_OK) return st;
     *bytes = footprint_of(*p);
+    return DBA_OK;
+}
+
+dba_status dba_device_memory(dba_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes) {
+    if (!ctx || !free_bytes || !total_bytes) return fail(DBA_ERR_INVALID, "null argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    size_t f = 0, t = 0;
+    HIP_TRY(hipMemGetInfo(&f, &t));
+    *free_bytes = (int64_t)f;
+    *total_bytes = (int64_t)t;
     return DBA_OK;
 }
 
@@ -628,7 +710,7 @@ dba_status dba_instance_create(dba_ctx* ctx, const dba_params* pp, int32_t* id) 
 
 #define ALLOC(ptr, count)                         \
     do {                                          \
-        dba_status s_ = dalloc(in, &(ptr), (count)); \
+        dba_status s_ = dalloc(in, &(ptr), (count), ctx->setup); \
         if (s_ != DBA_OK) { free_instance(in); delete in; return s_; } \
     } while (0)
     for (int f = 0; f < L.nf; f++) {
@@ -656,11 +738,13 @@ dba_status dba_instance_create(dba_ctx* ctx, const dba_params* pp, int32_t* id) 
     double scal[16] = {0};
     scal[1] = p.dt;
     for (int s = 0; s < L.nsets; s++)
-        HIP_TRY(hipMemcpy(h.a_cur[s], all.data(), sizeof(uint64_t) * L.nwords, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(h.scal, scal, sizeof(scal), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(in->d_self, &h, sizeof(Inst), hipMemcpyHostToDevice));
-    // the null-stream memsets/copies above must land before work on ctx->stream
-    HIP_TRY(hipDeviceSynchronize());
+        HIP_TRY(hipMemcpyAsync(h.a_cur[s], all.data(), sizeof(uint64_t) * L.nwords, hipMemcpyHostToDevice,
+                               ctx->setup));
+    HIP_TRY(hipMemcpyAsync(h.scal, scal, sizeof(scal), hipMemcpyHostToDevice, ctx->setup));
+    HIP_TRY(hipMemcpyAsync(in->d_self, &h, sizeof(Inst), hipMemcpyHostToDevice, ctx->setup));
+    // set-up lands before any later work on the compute stream (host-side wait);
+    // work already running there for other instances is not waited for
+    HIP_TRY(hipStreamSynchronize(ctx->setup));
     in->footprint = footprint_of(p);
     in->ncell = L.n;
     in->id = ctx->next_id++;
@@ -709,15 +793,18 @@ dba_status dba_upload(dba_ctx* ctx, int32_t id, int32_t field, const double* hos
     if (!in) return fail(DBA_ERR_NOTFOUND, "unknown instance %d", id);
     if (field < 0 || field >= in->h.nfields) return fail(DBA_ERR_INVALID, "field %d invalid", field);
     if ((int64_t)n != in->ncell) return fail(DBA_ERR_INVALID, "n=%zu, expected %lld", n, (long long)in->ncell);
-    HIP_TRY(hipMemcpyAsync(in->h.buf[0][field], host, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(in->h.buf[1][field], in->h.buf[0][field], n * sizeof(double), hipMemcpyDeviceToDevice,
-                           ctx->stream));
+    // An instance that was never stepped is touched by no queued kernel, so its
+    // upload overlaps the steps of other instances; otherwise it is ordered
+    // after them on the compute stream.
+    hipStream_t s = in->stepped ? ctx->stream : ctx->setup;
+    HIP_TRY(hipMemcpyAsync(in->h.buf[0][field], host, n * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(in->h.buf[1][field], in->h.buf[0][field], n * sizeof(double), hipMemcpyDeviceToDevice, s));
     if (in->h.ncons > 0) {
-        hipLaunchKernelGGL(reset_epochs_kernel, dim3(64), dim3(256), 0, ctx->stream, in->d_self, 0);
+        hipLaunchKernelGGL(reset_epochs_kernel, dim3(64), dim3(256), 0, s, in->d_self, 0);
         LAUNCH_CHECK();
     }
     if (in->p.model == DBA_MODEL_KH) in->needs_prime = true;
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipStreamSynchronize(s));
     return DBA_OK;
 }
 
@@ -759,6 +846,7 @@ dba_status dba_step_graph(dba_ctx* ctx, const int32_t* ids, int32_t n_ids, int32
         auto it = ctx->graphs.find(key);
         hipGraphExec_t exec = nullptr;
         if (it == ctx->graphs.end()) {
+            evict_graphs(ctx);
             hipGraph_t graph = nullptr;
             HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
             dba_status est = enqueue_steps(ctx, gs, chunk, false);
@@ -767,9 +855,10 @@ dba_status dba_step_graph(dba_ctx* ctx, const int32_t* ids, int32_t n_ids, int32
             if (e != hipSuccess) return fail(DBA_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
             HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
             (void)hipGraphDestroy(graph);
-            ctx->graphs[key] = exec;
+            ctx->graphs[key] = CachedGraph{exec, ctx->clock};
         } else {
-            exec = it->second;
+            it->second.last_use = ctx->clock;
+            exec = it->second.exec;
         }
         for (int r = 0; r < reps; r++) HIP_TRY(hipGraphLaunch(exec, ctx->stream));
     }

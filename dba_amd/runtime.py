@@ -100,6 +100,12 @@ class Context:
         abi.check(self.lib.dba_footprint(C.byref(params), C.byref(b)))
         return b.value
 
+    def device_memory(self):
+        """(free, total) bytes of the context's GPU."""
+        f, t = C.c_int64(), C.c_int64()
+        abi.check(self.lib.dba_device_memory(self.h, C.byref(f), C.byref(t)))
+        return f.value, t.value
+
     def step(self, instances, n=1, graph_chunk=0):
         ids = (C.c_int32 * len(instances))(*[i.id for i in instances])
         if graph_chunk and graph_chunk > 0:
