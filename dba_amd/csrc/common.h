@@ -212,4 +212,36 @@ __device__ __forceinline__ int find_slot(const int32_t* offsets, int n, int item
     return lo;
 }
 
+// What a step kernel iterates over: the group's merged worklist. Groups of a
+// single instance with a single segment also pass that instance's list and
+// tile count, so the list entry, the list length and the descriptor are
+// fetched in one round of loads instead of a dependent chain.
+struct WorkList {
+    const Inst* insts;
+    const Seg* segs;
+    const int32_t* offsets;   // exclusive prefix of segment counts, [nseg] = total
+    int nseg;
+    const int32_t* solo_seg;  // single-instance single-segment groups: its list, else null
+    int solo_nt;              // ... and its tile count (0 otherwise)
+};
+
+// Item -> (instance, tile); false once the item is past the list. `total` is
+// offsets[nseg], loaded by the caller once per workgroup.
+__device__ __forceinline__ bool next_item(const WorkList& w, int item, int total, const Inst** I, int* t) {
+    if (w.solo_nt > 0) {
+        if (item >= w.solo_nt) return false;
+        const int tt = w.solo_seg[item];  // in bounds; loaded before the length check so both loads overlap
+        if (item >= total) return false;
+        *I = w.insts;
+        *t = tt;
+        return true;
+    }
+    if (item >= total) return false;
+    const int s = w.nseg == 1 ? 0 : find_slot(w.offsets, w.nseg, item);
+    const Seg sg = w.segs[s];
+    *I = w.insts + sg.inst;
+    *t = (*I)->seg[sg.tile0 + (item - w.offsets[s])];
+    return true;
+}
+
 }  // namespace dba

@@ -99,6 +99,8 @@ struct Group {
     int64_t total_tiles = 0;
     int grid = 0;
     uint64_t last_use = 0;
+    int32_t* solo_seg = nullptr;  // one instance in one segment: its worklist ...
+    int solo_nt = 0;              // ... and tile count
 };
 
 struct GraphKey {
@@ -385,6 +387,10 @@ dba_status get_group(dba_ctx* ctx, const std::vector<int>& ids, Group** out) {
     }
     g->nseg = (int)segs.size();
     Instance* first = ctx->inst[ids[0]];
+    if (ids.size() == 1 && g->nseg == 1) {
+        g->solo_seg = first->h.seg;
+        g->solo_nt = first->h.nt;
+    }
     g->model = first->p.model;
     g->tile = first->p.tile_x;
     g->tile_z = first->p.tile_z;
@@ -415,6 +421,10 @@ dba_status get_group(dba_ctx* ctx, const std::vector<int>& ids, Group** out) {
 
 // ---------------------------------------------------------------- launches
 
+WorkList wl_of(const Group* g) {
+    return WorkList{g->d_insts, g->d_segs, g->d_offsets, g->nseg, g->solo_seg, g->solo_nt};
+}
+
 template <int TB, int NT>
 void launch_step_tb(const Group* g, hipStream_t s) {
     dim3 grid(g->grid), block(NT);
@@ -422,18 +432,15 @@ void launch_step_tb(const Group* g, hipStream_t s) {
         case DBA_MODEL_CH: {
             // the strip layout needs NT <= TB*TB
             constexpr int NTC = NT <= TB * TB ? NT : TB * TB;
-            hipLaunchKernelGGL((ch_step_kernel<TB, NTC>), grid, dim3(NTC), 0, s, g->d_insts, g->d_segs,
-                               g->d_offsets, g->nseg);
+            hipLaunchKernelGGL((ch_step_kernel<TB, NTC>), grid, dim3(NTC), 0, s, wl_of(g));
             break;
         }
         case DBA_MODEL_DENDRITE_K93:
-            hipLaunchKernelGGL((k93_step_kernel<TB, NT>), grid, block, 0, s, g->d_insts, g->d_segs, g->d_offsets,
-                               g->nseg);
+            hipLaunchKernelGGL((k93_step_kernel<TB, NT>), grid, block, 0, s, wl_of(g));
             break;
         case DBA_MODEL_KH:
             if constexpr (TB <= 16) {
-                hipLaunchKernelGGL((kh_step_kernel<TB, NT>), grid, block, 0, s, g->d_insts, g->d_segs, g->d_offsets,
-                                   g->nseg);
+                hipLaunchKernelGGL((kh_step_kernel<TB, NT>), grid, block, 0, s, wl_of(g));
             }
             break;
         default:
@@ -455,18 +462,14 @@ void launch_step(const Group* g, hipStream_t s) {
         dim3 grid(g->grid), block(64);
         if (g->model == DBA_MODEL_MODELB) {
             if (g->tile_z == 4)
-                hipLaunchKernelGGL((modelb_step_kernel<4, 4>), grid, block, 0, s, g->d_insts, g->d_segs,
-                                   g->d_offsets, g->nseg);
+                hipLaunchKernelGGL((modelb_step_kernel<4, 4>), grid, block, 0, s, wl_of(g));
             else
-                hipLaunchKernelGGL((modelb_step_kernel<4, 1>), grid, block, 0, s, g->d_insts, g->d_segs,
-                                   g->d_offsets, g->nseg);
+                hipLaunchKernelGGL((modelb_step_kernel<4, 1>), grid, block, 0, s, wl_of(g));
         } else {
             if (g->tile_z == 4)
-                hipLaunchKernelGGL((k94_step_kernel<4, 4>), grid, block, 0, s, g->d_insts, g->d_segs, g->d_offsets,
-                                   g->nseg);
+                hipLaunchKernelGGL((k94_step_kernel<4, 4>), grid, block, 0, s, wl_of(g));
             else
-                hipLaunchKernelGGL((k94_step_kernel<4, 1>), grid, block, 0, s, g->d_insts, g->d_segs, g->d_offsets,
-                                   g->nseg);
+                hipLaunchKernelGGL((k94_step_kernel<4, 1>), grid, block, 0, s, wl_of(g));
         }
         return;
     }

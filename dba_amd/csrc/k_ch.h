@@ -159,9 +159,7 @@ __device__ __forceinline__ void ch_tile(const Inst& I, int t, NbTable& nb, doubl
 }
 
 template <int TB, int NT>
-__global__ __launch_bounds__(NT) void ch_step_kernel(const Inst* __restrict__ insts,
-                                                    const Seg* __restrict__ segs,
-                                                    const int32_t* __restrict__ offsets, int nseg) {
+__global__ __launch_bounds__(NT) void ch_step_kernel(const WorkList wl) {
     static_assert(NT % TB == 0 && TB * TB % NT == 0, "thread layout needs TB | NT | TB*TB");
     constexpr int W2 = TB + 4;
     constexpr int W1 = TB + 2;
@@ -171,13 +169,13 @@ __global__ __launch_bounds__(NT) void ch_step_kernel(const Inst* __restrict__ in
     __shared__ int sflag[NT / 64 + 1];
     __shared__ NbTable nb;
 
-    const int total = offsets[nseg];
+    const int total = wl.offsets[wl.nseg];
     const int G = gridDim.x;
-    for (int item = xcd_remap(blockIdx.x, G); item < total; item += G) {
-        const int s = nseg == 1 ? 0 : find_slot(offsets, nseg, item);
-        const Seg sg = segs[s];
-        const Inst& I = insts[sg.inst];
-        const int t = I.seg[sg.tile0 + (item - offsets[s])];
+    for (int item = xcd_remap(blockIdx.x, G);; item += G) {
+        const Inst* Ip;
+        int t;
+        if (!next_item(wl, item, total, &Ip, &t)) break;
+        const Inst& I = *Ip;
         if (I.stencil != 5) ch_tile<TB, NT, true>(I, t, nb, sc, smu, sred, sflag);
         else ch_tile<TB, NT, false>(I, t, nb, sc, smu, sred, sflag);
     }

@@ -129,7 +129,8 @@ constexpr int kSegWords = kSegTiles / 64;
 // Per-instance end-of-step update from the summed segment partials.
 __device__ __forceinline__ void finish_instance(const Inst& J, int64_t listlen, unsigned long long c0,
                                                 unsigned long long c1, unsigned long long cells, const double* cs,
-                                                double vmax, int64_t step, const double* scal_in, int rotate) {
+                                                double vmax, int64_t step, int64_t updates, const double* scal_in,
+                                                int rotate) {
     J.ctr[2] = listlen;
     if (!rotate) return;
     if (step < J.hist_cap) {
@@ -137,7 +138,7 @@ __device__ __forceinline__ void finish_instance(const Inst& J, int64_t listlen, 
         J.hist[2 * step + 1] = J.nsets > 1 ? (int32_t)c1 : 0;
     }
     J.ctr[0] = step + 1;
-    J.ctr[1] += (int64_t)cells;
+    J.ctr[1] = updates + (int64_t)cells;
     for (int f = 0; f < J.ncons; f++) J.scal[8 + f] = scal_in[8 + f] + cs[f] / J.ntotal;
     const double used = scal_in[1];
     J.scal[2] = scal_in[2] + used;
@@ -156,6 +157,7 @@ struct FinLds {
     double s_cs[kMaxFields];
     double s_scal[16];
     int64_t s_step;
+    int64_t s_upd;
     int s_last;
 };
 
@@ -177,6 +179,7 @@ __device__ __forceinline__ void finalize_body(const Inst* __restrict__ insts, co
     auto& s_cs = F.s_cs;
     auto& s_scal = F.s_scal;
     auto& s_step = F.s_step;
+    auto& s_upd = F.s_upd;
     auto& s_last = F.s_last;
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -223,6 +226,7 @@ __device__ __forceinline__ void finalize_body(const Inst* __restrict__ insts, co
     }
     if (solo && tid < 16) s_scal[tid] = I.scal[tid];
     if (solo && tid == 16) s_step = I.ctr[0];
+    if (solo && tid == 17) s_upd = I.ctr[1];
     if (tid < 3) s_cnt[tid] = 0ull;
     if (wthread) s_cur[ws][wj] = cur;
     else if (tid < 2 * kSegWords) s_cur[ws][wj] = 0ull;
@@ -316,7 +320,8 @@ __device__ __forceinline__ void finalize_body(const Inst* __restrict__ insts, co
     if (solo) {
         if (tid == 0) {
             double cs[kMaxFields] = {s_cs[0], s_cs[1], s_cs[2], s_cs[3]};
-            finish_instance(I, s_pre[kSegWords], s_cnt[0], s_cnt[1], s_cnt[2], cs, vmax, s_step, s_scal, rotate);
+            finish_instance(I, s_pre[kSegWords], s_cnt[0], s_cnt[1], s_cnt[2], cs, vmax, s_step, s_upd, s_scal,
+                            rotate);
             offsets[0] = 0;
             offsets[1] = s_pre[kSegWords];
         }
@@ -357,7 +362,7 @@ __device__ __forceinline__ void finalize_body(const Inst* __restrict__ insts, co
         }
         double sc[16];
         for (int k = 0; k < 16; k++) sc[k] = J.scal[k];
-        finish_instance(J, listlen, c0, c1, cells, cs, vm, J.ctr[0], sc, rotate);
+        finish_instance(J, listlen, c0, c1, cells, cs, vm, J.ctr[0], J.ctr[1], sc, rotate);
     }
     int running = 0;
     for (int base = 0; base < nseg; base += NT) {

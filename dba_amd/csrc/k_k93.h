@@ -42,9 +42,7 @@ __device__ __forceinline__ void k93_aniso(const Inst& I, double gx, double gy, d
 }
 
 template <int TB, int NT>
-__global__ __launch_bounds__(NT) void k93_step_kernel(const Inst* __restrict__ insts,
-                                                     const Seg* __restrict__ segs,
-                                                     const int32_t* __restrict__ offsets, int nseg) {
+__global__ __launch_bounds__(NT) void k93_step_kernel(const WorkList wl) {
     constexpr int W2 = TB + 4;
     constexpr int W1 = TB + 2;
     __shared__ double sp[W2 * W2];
@@ -55,14 +53,14 @@ __global__ __launch_bounds__(NT) void k93_step_kernel(const Inst* __restrict__ i
     __shared__ int sflag[NT / 64 + 1];
     __shared__ NbTable nb;
 
-    const int total = offsets[nseg];
+    const int total = wl.offsets[wl.nseg];
     const int G = gridDim.x;
     const int tid = threadIdx.x;
-    for (int item = xcd_remap(blockIdx.x, G); item < total; item += G) {
-        const int s = nseg == 1 ? 0 : find_slot(offsets, nseg, item);
-        const Seg sg = segs[s];
-        const Inst& I = insts[sg.inst];
-        const int t = I.seg[sg.tile0 + (item - offsets[s])];
+    for (int item = xcd_remap(blockIdx.x, G);; item += G) {
+        const Inst* Ip;
+        int t;
+        if (!next_item(wl, item, total, &Ip, &t)) break;
+        const Inst& I = *Ip;
         const int nx = I.nx, ny = I.ny, ntx = I.ntx, bcx = I.bc_x, bcy = I.bc_y;
         const int x0 = (t % ntx) * TB, y0 = (t / ntx) * TB;
         const bool regular = I.mode == 0;

@@ -74,9 +74,7 @@ __device__ __forceinline__ int nb_dir3(int lx, int ly, int lz) {
 
 // ------------------------------------------------------------ Model B (ternary)
 template <int TB, int TZ>
-__global__ __launch_bounds__(64) void modelb_step_kernel(const Inst* __restrict__ insts,
-                                                        const Seg* __restrict__ segs,
-                                                        const int32_t* __restrict__ offsets, int nseg) {
+__global__ __launch_bounds__(64) void modelb_step_kernel(const WorkList wl) {
     constexpr int NT = 64;
     constexpr int W2X = TB + 4, W2Z = TZ + 4;  // x, y, z with a 2-cell halo
     constexpr int W1X = TB + 2, W1Z = TZ + 2;  // mu with a 1-cell halo
@@ -88,14 +86,14 @@ __global__ __launch_bounds__(64) void modelb_step_kernel(const Inst* __restrict_
     __shared__ int sflag[2];
     __shared__ NbTable3 nb;
 
-    const int total = offsets[nseg];
+    const int total = wl.offsets[wl.nseg];
     const int G = gridDim.x;
     const int tid = threadIdx.x;
-    for (int item = xcd_remap(blockIdx.x, G); item < total; item += G) {
-        const int s = nseg == 1 ? 0 : find_slot(offsets, nseg, item);
-        const Seg sg = segs[s];
-        const Inst& I = insts[sg.inst];
-        const int t = I.seg[sg.tile0 + (item - offsets[s])];
+    for (int item = xcd_remap(blockIdx.x, G);; item += G) {
+        const Inst* Ip;
+        int t;
+        if (!next_item(wl, item, total, &Ip, &t)) break;
+        const Inst& I = *Ip;
         const int nx = I.nx, ny = I.ny, nz = I.nz;
         const int x0 = (t % I.ntx) * TB, y0 = ((t / I.ntx) % I.nty) * TB, z0 = (t / (I.ntx * I.nty)) * TZ;
         const bool regular = I.mode == 0;
@@ -198,9 +196,7 @@ __global__ __launch_bounds__(64) void modelb_step_kernel(const Inst* __restrict_
 
 // ------------------------------------------------------------ Kobayashi-94 dendrite
 template <int TB, int TZ>
-__global__ __launch_bounds__(64) void k94_step_kernel(const Inst* __restrict__ insts,
-                                                     const Seg* __restrict__ segs,
-                                                     const int32_t* __restrict__ offsets, int nseg) {
+__global__ __launch_bounds__(64) void k94_step_kernel(const WorkList wl) {
     constexpr int NT = 64;
     constexpr int WX = TB + 2, WZ = TZ + 2;
     constexpr int A = WX * WX * WZ;
@@ -209,14 +205,14 @@ __global__ __launch_bounds__(64) void k94_step_kernel(const Inst* __restrict__ i
     __shared__ int sflag[2];
     __shared__ NbTable3 nb;
 
-    const int total = offsets[nseg];
+    const int total = wl.offsets[wl.nseg];
     const int G = gridDim.x;
     const int tid = threadIdx.x;
-    for (int item = xcd_remap(blockIdx.x, G); item < total; item += G) {
-        const int s = nseg == 1 ? 0 : find_slot(offsets, nseg, item);
-        const Seg sg = segs[s];
-        const Inst& I = insts[sg.inst];
-        const int t = I.seg[sg.tile0 + (item - offsets[s])];
+    for (int item = xcd_remap(blockIdx.x, G);; item += G) {
+        const Inst* Ip;
+        int t;
+        if (!next_item(wl, item, total, &Ip, &t)) break;
+        const Inst& I = *Ip;
         const int nx = I.nx, ny = I.ny, nz = I.nz;
         const int x0 = (t % I.ntx) * TB, y0 = ((t / I.ntx) % I.nty) * TB, z0 = (t / (I.ntx * I.nty)) * TZ;
         const bool regular = I.mode == 0;

@@ -1,0 +1,123 @@
+"""coarsening_analysis (dba_amd/bin): droplet sizes and mean radius of CH /
+Model B outputs, checked against an independent labelling
+(scipy.ndimage.label, then a union of the labels that touch across each
+periodic face) and the reference's radius formulas."""
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+from scipy import ndimage
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TOOL = os.path.join(ROOT, "dba_amd", "bin", "coarsening_analysis")
+
+
+def tool():
+    if not os.path.exists(TOOL):
+        pytest.skip("coarsening_analysis not built (run __graft_entry__.build())")
+    return TOOL
+
+
+def write_vtk(path, arrays, shape):
+    nz, ny, nx = shape
+    with open(path, "w") as f:
+        f.write("# vtk DataFile Version 2.0\nOUTPUT by Roy Zhang\nASCII\nDATASET RECTILINEAR_GRID\n")
+        f.write(f"DIMENSIONS {nx} {ny} {nz}\n")
+        for name, n in (("X", nx), ("Y", ny), ("Z", nz)):
+            f.write(f"{name}_COORDINATES {n} int\n" + "".join(f"{i}\t" for i in range(n)) + "\n")
+        f.write(f"POINT_DATA {nx * ny * nz}\n")
+        for k, (name, a) in enumerate(arrays):
+            if k:
+                f.write("\n")
+            f.write(f"SCALARS {name} double\nLOOKUP_TABLE default\n")
+            f.write(" ".join(repr(float(v)) for v in a.ravel()) + " ")
+        f.write("\n")
+
+
+def periodic_sizes(a, avg):
+    """Sizes of periodic face-connected components of a >= avg that contain a cell > avg."""
+    mask = a >= avg
+    lab, n = ndimage.label(mask)
+    parent = list(range(n + 1))
+
+    def root(i):
+        while parent[i] != i:
+            parent[i] = parent[parent[i]]
+            i = parent[i]
+        return i
+
+    for ax in range(a.ndim):
+        if a.shape[ax] == 1:
+            continue
+        lo = np.take(lab, 0, axis=ax)
+        hi = np.take(lab, -1, axis=ax)
+        for u, v in zip(lo.ravel(), hi.ravel()):
+            if u and v:
+                parent[root(u)] = root(v)
+    sizes = {}
+    seeds = set()
+    flat = lab.ravel()
+    for idx, l in enumerate(flat):
+        if l:
+            r = root(l)
+            sizes[r] = sizes.get(r, 0) + 1
+            if a.ravel()[idx] > avg:
+                seeds.add(r)
+    return sorted(sizes[r] for r in seeds)
+
+
+def threshold(a):
+    return (max(0.5, a.max()) + min(0.5, a.min())) / 2.0
+
+
+def run_tool(tmp_path, args):
+    r = subprocess.run([tool(), *args], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    sizes = [sorted(int(x) for x in line.split()) for line in (tmp_path / "size_dist.txt").read_text().splitlines()]
+    radii = [float(line.split(",")[1]) for line in (tmp_path / "avg_radius.txt").read_text().splitlines()]
+    return sizes, radii
+
+
+@pytest.mark.parametrize("shape", [(1, 64, 48), (12, 10, 14)])
+def test_sizes_and_radius_match_independent_labelling(tmp_path, shape):
+    rng = np.random.default_rng(3)
+    files = []
+    expect_sizes, expect_r = [], []
+    for k in range(3):
+        # smooth periodic field so droplets span the faces
+        a = ndimage.uniform_filter(rng.random(shape), size=3, mode="wrap") * 0.8 + 0.1 * k
+        avg = threshold(a)
+        files.append(f"output_{k * 100}.vtk")
+        write_vtk(tmp_path / files[-1], [("X", a * 0), ("Y", a * 0), ("Z", a)], shape)
+        sz = periodic_sizes(a, avg)
+        expect_sizes.append(sz)
+        vol = int((a > avg).sum())
+        if shape[0] > 1:
+            expect_r.append((vol * 0.75 / math.pi / len(sz)) ** (1.0 / 3.0))
+        else:
+            expect_r.append(math.sqrt(vol / math.pi / len(sz)))
+    sizes, radii = run_tool(tmp_path, ["-s", "100", "-n", "3"])
+    assert sizes == expect_sizes
+    assert np.allclose(radii, expect_r, rtol=1e-5)   # printed with 6 digits
+    assert sum(len(s) for s in sizes) > 3
+
+
+def test_field_selection_and_wrap(tmp_path):
+    """A droplet cut by both periodic edges of a 2D CH field counts once."""
+    c = -np.ones((1, 32, 32))
+    c[0, :3, :3] = c[0, -2:, :3] = c[0, :3, -2:] = c[0, -2:, -2:] = 1.0   # one droplet across the corners
+    c[0, 14:18, 14:18] = 1.0                                                # one in the middle
+    write_vtk(tmp_path / "a.vtk", [("C", c)], c.shape)
+    sizes, radii = run_tool(tmp_path, ["-f", "C", "a.vtk"])
+    assert sizes == [[16, 25]]
+    assert radii[0] == pytest.approx(math.sqrt(41 / math.pi / 2), rel=1e-5)
+
+
+def test_missing_file_and_field(tmp_path):
+    r = subprocess.run([tool(), "nope.vtk"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "can not open" in r.stdout
+    write_vtk(tmp_path / "b.vtk", [("C", np.zeros((1, 4, 4)))], (1, 4, 4))
+    r = subprocess.run([tool(), "-f", "Z", "b.vtk"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "no SCALARS Z" in r.stdout
