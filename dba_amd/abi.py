@@ -130,6 +130,9 @@ SIGNATURES = [
     ("dba_marker", C.c_int32, [C.c_void_p, C.c_int32]),
     ("dba_marker_elapsed", C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
     ("dba_check_finite", C.c_int32, [C.c_void_p, C.c_int32]),
+    ("dba_field_sum", C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
+    ("dba_growth_front", C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p]),
+    ("dba_kh_vy_mode", C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
 ]
 
 _lib = None

@@ -212,6 +212,20 @@ __device__ __forceinline__ int find_slot(const int32_t* offsets, int n, int item
     return lo;
 }
 
+// Current value of a field at cell i: the buffer the tile parity selects,
+// minus the conserved shift accumulated since the tile last ran.
+__device__ __forceinline__ double current_value(const Inst& I, int field, int64_t i) {
+    const int x = (int)(i % I.nx);
+    const int y = (int)((i / I.nx) % I.ny);
+    const int z = (int)(i / ((int64_t)I.nx * I.ny));
+    const int t = x / I.tx + (y / I.ty) * I.ntx + (z / I.tz) * I.ntx * I.nty;
+    const int set = I.field_set[field];
+    const int p = bit_of(I.par[set], t);
+    double v = I.buf[p][field][i];
+    if (field < I.ncons) v -= (I.scal[8 + field] - I.s_tile[((int64_t)field * 2 + p) * I.nt + t]);
+    return v;
+}
+
 // What a step kernel iterates over: the group's merged worklist. Groups of a
 // single instance with a single segment also pass that instance's list and
 // tile count, so the list entry, the list length and the descriptor are

@@ -25,6 +25,7 @@
 #include "k_k93.h"
 #include "k_kh.h"
 #include "k_ref3d.h"
+#include "k_diag.h"
 
 using namespace dba;
 
@@ -128,6 +129,8 @@ struct dba_ctx {
     double* staging = nullptr;
     size_t staging_n = 0;
     int* d_flag = nullptr;
+    double* d_diag = nullptr;      // diagnostics: block partials, results
+    int32_t* d_cols = nullptr;     // diagnostics: growth-front columns
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_step, ev_fin;
     std::vector<hipEvent_t> ev_pool;
@@ -621,6 +624,8 @@ dba_status dba_init(int32_t device, dba_ctx** out) {
         return fail(DBA_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
     }
     HIP_TRY(hipMalloc(&ctx->d_flag, sizeof(int)));
+    HIP_TRY(hipMalloc(&ctx->d_diag, sizeof(double) * (3 * kDiagBlocks + kMaxFrontCols + 8)));
+    HIP_TRY(hipMalloc(&ctx->d_cols, sizeof(int32_t) * kMaxFrontCols));
     for (auto& m : ctx->markers) HIP_TRY(hipEventCreate(&m));
     *out = ctx;
     return DBA_OK;
@@ -640,6 +645,8 @@ dba_status dba_finalize(dba_ctx* ctx) {
     for (auto m : ctx->markers) (void)hipEventDestroy(m);
     if (ctx->staging) (void)hipFree(ctx->staging);
     if (ctx->d_flag) (void)hipFree(ctx->d_flag);
+    if (ctx->d_diag) (void)hipFree(ctx->d_diag);
+    if (ctx->d_cols) (void)hipFree(ctx->d_cols);
     (void)hipStreamDestroy(ctx->stream);
     (void)hipStreamDestroy(ctx->setup);
     delete ctx;
@@ -985,6 +992,62 @@ dba_status dba_check_finite(dba_ctx* ctx, int32_t id) {
     HIP_TRY(hipMemcpyAsync(&bad, ctx->d_flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (bad) return fail(DBA_ERR_NUMERIC, "instance %d holds non-finite values", id);
+    return DBA_OK;
+}
+
+dba_status dba_field_sum(dba_ctx* ctx, int32_t id, int32_t field, double* sum) {
+    if (!ctx || !sum) return fail(DBA_ERR_INVALID, "null argument");
+    Instance* in = find(ctx, id);
+    if (!in) return fail(DBA_ERR_NOTFOUND, "unknown instance %d", id);
+    if (field < 0 || field >= in->h.nfields) return fail(DBA_ERR_INVALID, "field %d invalid", field);
+    double* part = ctx->d_diag;
+    double* res = ctx->d_diag + 3 * kDiagBlocks;
+    hipLaunchKernelGGL(field_sum_kernel, dim3(kDiagBlocks), dim3(kDiagNT), 0, ctx->stream, in->d_self, 0, field,
+                       part);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(reduce_parts_kernel, dim3(1), dim3(64), 0, ctx->stream, part, 1, res);
+    LAUNCH_CHECK();
+    HIP_TRY(hipMemcpyAsync(sum, res, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return DBA_OK;
+}
+
+dba_status dba_growth_front(dba_ctx* ctx, int32_t id, const int32_t* xs, int32_t n, double* front) {
+    if (!ctx || !xs || !front) return fail(DBA_ERR_INVALID, "null argument");
+    Instance* in = find(ctx, id);
+    if (!in) return fail(DBA_ERR_NOTFOUND, "unknown instance %d", id);
+    if (in->p.model != DBA_MODEL_DENDRITE_K93 && in->p.model != DBA_MODEL_DENDRITE_K94)
+        return fail(DBA_ERR_INVALID, "growth front needs a dendrite instance");
+    if (n < 0 || n > kMaxFrontCols) return fail(DBA_ERR_INVALID, "n must be in [0, %d]", kMaxFrontCols);
+    for (int k = 0; k < n; k++)
+        if (xs[k] < 0 || xs[k] >= in->p.nx) return fail(DBA_ERR_INVALID, "column %d outside the grid", xs[k]);
+    if (n == 0) return DBA_OK;
+    // synchronous so the host array may be reused at once; ordered after queued steps
+    HIP_TRY(hipMemcpyAsync(ctx->d_cols, xs, sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx->stream));
+    double* res = ctx->d_diag + 3 * kDiagBlocks;
+    hipLaunchKernelGGL(growth_front_kernel, dim3(1), dim3(kMaxFrontCols), 0, ctx->stream, in->d_self, 0,
+                       ctx->d_cols, n, res);
+    LAUNCH_CHECK();
+    HIP_TRY(hipMemcpyAsync(front, res, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return DBA_OK;
+}
+
+dba_status dba_kh_vy_mode(dba_ctx* ctx, int32_t id, double* amplitude) {
+    if (!ctx || !amplitude) return fail(DBA_ERR_INVALID, "null argument");
+    Instance* in = find(ctx, id);
+    if (!in) return fail(DBA_ERR_NOTFOUND, "unknown instance %d", id);
+    if (in->p.model != DBA_MODEL_KH) return fail(DBA_ERR_INVALID, "vy mode needs a KH instance");
+    double* part = ctx->d_diag;
+    double* res = ctx->d_diag + 3 * kDiagBlocks;
+    hipLaunchKernelGGL(vy_mode_kernel, dim3(kDiagBlocks), dim3(kDiagNT), 0, ctx->stream, in->d_self, 0, part);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(reduce_parts_kernel, dim3(1), dim3(64), 0, ctx->stream, part, 3, res);
+    LAUNCH_CHECK();
+    double s[3];
+    HIP_TRY(hipMemcpyAsync(s, res, sizeof(s), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    *amplitude = 2.0 * std::sqrt(s[0] * s[0] / (s[2] * s[2]) + s[1] * s[1] / (s[2] * s[2]));
     return DBA_OK;
 }
 

@@ -396,18 +396,8 @@ __global__ __launch_bounds__(kFinNT) void finalize_kernel(const Inst* __restrict
 __global__ void materialize_kernel(const Inst* __restrict__ insts, int slot, int field, double* __restrict__ dst) {
     const Inst& I = insts[slot];
     const int64_t n = (int64_t)I.nx * I.ny * I.nz;
-    const int set = I.field_set[field];
-    const bool cons = field < I.ncons;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const int x = (int)(i % I.nx);
-        const int y = (int)((i / I.nx) % I.ny);
-        const int z = (int)(i / ((int64_t)I.nx * I.ny));
-        const int t = x / I.tx + (y / I.ty) * I.ntx + (z / I.tz) * I.ntx * I.nty;
-        const int p = bit_of(I.par[set], t);
-        double v = I.buf[p][field][i];
-        if (cons) v -= (I.scal[8 + field] - I.s_tile[((int64_t)field * 2 + p) * I.nt + t]);
-        dst[i] = v;
-    }
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = current_value(I, field, i);
 }
 
 // After an upload: both buffers hold the new values, so the shift epochs of

@@ -6,6 +6,8 @@
 //   RTValues.out    "Time Vel Act_portion_p Act_portion_T Steps_per_second"
 //                   every t_freq steps (Calc_RealTime_Values, kernel.cu:797-855)
 //   tip_profile.out growth front at Nx/2-10 .. Nx/2+10 (tip_curvature, kernel.cu:776-790)
+// The growth front is evaluated on the device (dba_growth_front); whole fields
+// are downloaded only when VTK output is requested.
 // Optional "key value" lines after the reference groups:
 //   model k94|k93   k94 (default): the reference 3D model; k93: 2D Kobayashi 1993
 //                   (anisotropy j, latent heat K, theta0, T_eq; zero-flux walls)
@@ -16,21 +18,6 @@
 #include "host_common.h"
 
 using namespace host;
-
-// First y >= Ny/2 on column x (z = Nz/2) where phi < 0.5, interpolated
-// (growth_front, example_Dendritic_growth/kernel.cu:760-774). Returns 0 when the
-// column never drops below 0.5 (the reference leaves the value unset).
-static double growth_front(const std::vector<double>& phi, int ix, const Grid& g) {
-    int z = g.nz / 2;
-    for (int y = g.ny / 2; y < g.ny; y++) {
-        size_t i = (size_t)ix + (size_t)y * g.nx + (size_t)z * g.nx * g.ny;
-        if (phi[i] < 0.5) {
-            size_t im = i - (size_t)g.nx;
-            return (phi[im] - 0.5) / (phi[im] - phi[i]) + y - 1 - g.ny / 2;
-        }
-    }
-    return 0.0;
-}
 
 int main() {
     Clock total;
@@ -143,22 +130,28 @@ int main() {
         t += n;
         if (t % t_freq) continue;
         std::cout << "Time step: " << t << std::endl;
-        phi = sim.download(DBA_F_PHI);
+        // the growth front comes from the device; whole fields only for VTK output
+        std::vector<int32_t> cols;
+        for (int ix = Nx / 2 - 10; ix <= Nx / 2 + 10; ix++) cols.push_back(((ix % Nx) + Nx) % Nx);
+        cols.push_back(Nx / 2);
+        std::vector<double> front(cols.size());
+        check(dba_growth_front(sim.ctx, sim.id, cols.data(), (int32_t)cols.size(), front.data()), "dba_growth_front");
         int cp = 0, cT = 0;
         sim.awake(DBA_SET_MAIN, &cp);
         sim.awake(DBA_SET_T, &cT);
         double pp = (double)cp / sim.nt, pT = (double)cT / sim.nt;
         printf("awake_p: %f, awake_T: %f\n", pp, pT);
-        double dist = growth_front(phi, Nx / 2, g);
+        double dist = front.back();
         double vel = (dist - prev_dist) / t_freq;
         prev_dist = dist;
         std::ostringstream line;
         line << t << " " << vel << " " << pp << " " << pT << " " << (double)t_freq / sec;
         append_line("RTValues.out", "Time Vel Act_portion_p Act_portion_T Steps_per_second", line.str());
         std::ostringstream tip;
-        for (int ix = Nx / 2 - 10; ix <= Nx / 2 + 10; ix++) tip << growth_front(phi, ix, g) << " ";
+        for (size_t k = 0; k + 1 < front.size(); k++) tip << front[k] << " ";
         append_line("tip_profile.out", "", tip.str());
         if (vtk) {
+            phi = sim.download(DBA_F_PHI);
             T = sim.download(DBA_F_T);
             write_structured("p", t, g, phi.data());
             write_structured("T", t, g, T.data());

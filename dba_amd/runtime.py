@@ -72,6 +72,24 @@ class Instance:
     def check_finite(self):
         return self.ctx.lib.dba_check_finite(self.ctx.h, self.id) == abi.OK
 
+    def field_sum(self, field):
+        """Sum of the field's current values, reduced on the device in a fixed order."""
+        out = C.c_double()
+        abi.check(self.ctx.lib.dba_field_sum(self.ctx.h, self.id, field, C.byref(out)))
+        return out.value
+
+    def growth_front(self, xs):
+        """Dendrite growth front at the given columns (at most 64)."""
+        xs = np.ascontiguousarray(xs, dtype=np.int32)
+        out = np.zeros(len(xs))
+        abi.check(self.ctx.lib.dba_growth_front(self.ctx.h, self.id, xs.ctypes.data, len(xs), out.ctypes.data))
+        return out
+
+    def kh_vy_mode(self):
+        out = C.c_double()
+        abi.check(self.ctx.lib.dba_kh_vy_mode(self.ctx.h, self.id, C.byref(out)))
+        return out.value
+
     def destroy(self):
         if self.id is not None:
             abi.check(self.ctx.lib.dba_instance_destroy(self.ctx.h, self.id))

@@ -12,6 +12,17 @@
  *             single-mode linear growth factor, conserved-DBA total, fixed points.
  *   k93_*     pinned: fixed points, frozen-phi heat-equation modes, hand-computed
  *             single-cell update, anisotropy vs atan2/cos/sin, D4 symmetry.
+ *   kh_*      pinned: bit-exact against the compiled reference CPU program
+ *             (KH_CPU.cpp) on two cases; uniform fixed point; conservation with
+ *             periodic and wall boundaries; transpose symmetry; 2nd-order acoustics.
+ *   mb_*      pinned: bit-exact against the compiled reference CPU program
+ *             (base_CPU.cpp) on an 8^3 case and a 2D plane; uniform fixed point.
+ *   k94_*     pinned to float32 resolution against the compiled (float32)
+ *             reference CPU program on the central block.
+ *   DBA modes (all models): always-awake equals regular bitwise, a tile that
+ *             never fires stays frozen bitwise, conserved totals invariant; the
+ *             stale-neighbour reads follow the reference's CUDA kernels, which
+ *             cannot run here, so they are only partially pinned.
  *   dilate / compact  pinned: brute force over all small flag patterns.
  */
 #define _GNU_SOURCE
