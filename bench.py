@@ -50,6 +50,7 @@ WORKLOADS = {
     "kh": dict(model="kh", n=2048, tile=16, mode="dba", b_alg=64, warm=100, steps=1000),
     "batch": dict(model="k93", b_alg=32, warm=0, steps=5000),
     "serve": dict(model="mix", b_alg=None, warm=0, steps=0),
+    "copycal": dict(model="ch", b_alg=None, warm=0, steps=0),
 }
 
 
@@ -126,6 +127,33 @@ def kernel_roofline(ctx, insts, b_alg, steps):
         "kernel": "step kernel (fused stencil + detector)", "b_alg_per_cell": b_alg,
         "avg_kernel_us": avg_k_s * 1e6, "avg_finalize_us": f_ms * 1e3 / max(1, f_n), "launches_timed": k_n,
     }
+
+
+def copy_calibration(n=4096, reps=8):
+    """Device-to-device copies of known size (the D2D half of dba_upload, the
+    runtime's blit kernel), for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    from dba_amd import Context, configs
+    with Context(0) as ctx:
+        inst = ctx.create(configs.ch_params(n=n, tile=32))
+        c = configs.ch_ic(n, n)
+        for _ in range(reps):
+            inst.upload(0, c)
+        inst.destroy()
+    return {"workload": "copycal", "copies": reps, "bytes_per_copy": n * n * 8,
+            "kernel": "__amd_rocclr_copyBuffer (D2D half of each upload)"}
+
+
+def traffic_from_profile(workload):
+    """HBM bytes per step-kernel launch measured by the rocprofv3 --pmc passes
+    committed in profiles/ (FETCH_SIZE + WRITE_SIZE, scaled by the copy
+    calibration), or None when no such measurement is committed."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def bench_batch(args, K):
@@ -275,6 +303,9 @@ def main():
     if args.workload == "serve":
         print(json.dumps(bench_serve(args)))
         return
+    if args.workload == "copycal":
+        print(json.dumps(copy_calibration()))
+        return
     p, fields = make(args.workload)
     ncell = p.nx * p.ny
 
@@ -342,7 +373,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic_from_profile(args.workload),
             "kernel": "step kernel (fused stencil + detector)",
             "b_alg_per_cell": w["b_alg"],
             "avg_kernel_us": avg_k_s * 1e6,

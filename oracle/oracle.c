@@ -889,6 +889,7 @@ int ora_history(ora_sim* s, int32_t* out, int cap) {
 typedef struct {
     ora_sim* s;
     int y0, y1, nsteps;
+    int tid, nth;
     pthread_barrier_t* bar;
 } slab_arg;
 
@@ -897,9 +898,74 @@ static void slab_copy(ora_sim* s, int nf, int y0, int y1) {
         memcpy(s->f[k] + (long)y0 * s->nx, s->fn[k] + (long)y0 * s->nx, sizeof(double) * (size_t)(y1 - y0) * s->nx);
 }
 
+/* KH regular step over a y-slab of a thread team: the phases of kh_step with
+   barriers between them; the ghost fills and the CFL reduction are done by
+   thread 0 (y0 == 0). Every cell sees the same operations as in kh_step. */
+static double* kh_slab_vmax;  /* per-thread maxima, indexed by thread */
+
+static void kh_step_slab(ora_sim* s, int y0, int y1, int tid, int nth, pthread_barrier_t* bar) {
+    for (int y = y0; y < y1; y++)
+        for (int x = 0; x < s->nx; x++) {
+            long i = (long)x + (long)y * s->nx, c = pidx(s, x, y);
+            kh_prims_of(s, i, &s->w[0][c], &s->w[1][c], &s->w[2][c], &s->w[3][c]);
+        }
+    if (s->p.dt_mode == DBA_DT_CFL) {
+        double vmax = 0.0;
+        for (int y = y0; y < y1; y++)
+            for (int x = 0; x < s->nx; x++) {
+                long c = pidx(s, x, y);
+                double sp = kh_speed(s, s->w[0][c], s->w[1][c], s->w[2][c], s->w[3][c]);
+                if (sp > vmax) vmax = sp;
+            }
+        kh_slab_vmax[tid] = vmax;
+    }
+    pthread_barrier_wait(bar);
+    if (tid == 0) {
+        kh_fill_prim_ghosts(s);
+        if (s->p.dt_mode == DBA_DT_CFL) {
+            double vmax = 0.0;
+            for (int k = 0; k < nth; k++)
+                if (kh_slab_vmax[k] > vmax) vmax = kh_slab_vmax[k];
+            s->dt_kh = s->p.cfl * (s->p.dx / vmax);
+        } else {
+            s->dt_kh = s->p.dt;
+        }
+    }
+    pthread_barrier_wait(bar);
+    for (int y = y0; y < y1; y++)
+        for (int x = 0; x < s->nx; x++) kh_grads_at(s, x, y);
+    pthread_barrier_wait(bar);
+    if (tid == 0) kh_fill_grad_ghosts(s);
+    pthread_barrier_wait(bar);
+    int wall = s->p.bc_y == DBA_BC_WALL;
+    for (int y = y0; y < y1; y++)
+        for (int x = 0; x < s->nx; x++) {
+            kh_face_fluxes(s, x, y, 1, 1);
+            if (wall && y == 0) kh_face_fluxes(s, x, -1, 0, 1);
+        }
+    pthread_barrier_wait(bar);
+    double dtdx = s->dt_kh * s->p.dx;
+    for (int y = y0; y < y1; y++)
+        for (int x = 0; x < s->nx; x++) {
+            long i = (long)x + (long)y * s->nx;
+            int xm = map_i(x - 1, s->nx, s->p.bc_x);
+            long c = pidx(s, x, y), cxm = pidx(s, xm, y);
+            long cym = (wall && y == 0) ? pidx(s, x, -1) : pidx(s, x, map_i(y - 1, s->ny, s->p.bc_y));
+            for (int q = 0; q < 4; q++)
+                s->f[q][i] += dtdx * (-1.0 * s->w[16 + q][c] + s->w[16 + q][cxm] - 1.0 * s->w[20 + q][c] + s->w[20 + q][cym]);
+        }
+}
+
 static void* slab_worker(void* argp) {
     slab_arg* a = (slab_arg*)argp;
     ora_sim* s = a->s;
+    if (s->p.model == DBA_MODEL_KH) {
+        for (int t = 0; t < a->nsteps; t++) {
+            kh_step_slab(s, a->y0, a->y1, a->tid, a->nth, a->bar);
+            pthread_barrier_wait(a->bar);
+        }
+        return NULL;
+    }
     for (int t = 0; t < a->nsteps; t++) {
         if (s->p.model == DBA_MODEL_CH) {
             ch_calc_mu(s, a->y0, a->y1);
@@ -921,8 +987,8 @@ static void* slab_worker(void* argp) {
 
 double ora_bench(ora_sim* s, int nsteps, int nthreads) {
     if (s->p.mode != DBA_MODE_REGULAR) return -1.0;
-    if (s->p.model == DBA_MODEL_KH) {
-        /* KH: serial step loop (ghost fills and the CFL maximum are global) */
+    if (s->p.model == DBA_MODEL_KH && nthreads <= 1) {
+        /* serial: the oracle's own step */
         struct timespec a, b;
         clock_gettime(CLOCK_MONOTONIC, &a);
         for (int k = 0; k < nsteps; k++) kh_step(s);
@@ -930,8 +996,11 @@ double ora_bench(ora_sim* s, int nsteps, int nthreads) {
         s->steps += nsteps;
         return (double)(b.tv_sec - a.tv_sec) + 1e-9 * (double)(b.tv_nsec - a.tv_nsec);
     }
+    if (s->p.model != DBA_MODEL_KH && s->p.model != DBA_MODEL_CH && s->p.model != DBA_MODEL_DENDRITE_K93)
+        return -1.0;
     if (nthreads < 1) nthreads = 1;
     if (nthreads > s->ny) nthreads = s->ny;
+    if (nthreads > 256) nthreads = 256;
     struct timespec t0, t1;
     pthread_barrier_t bar;
     pthread_barrier_init(&bar, NULL, (unsigned)nthreads);
@@ -943,7 +1012,11 @@ double ora_bench(ora_sim* s, int nsteps, int nthreads) {
         args[k].y1 = (int)((long)s->ny * (k + 1) / nthreads);
         args[k].nsteps = nsteps;
         args[k].bar = &bar;
+        args[k].tid = k;
+        args[k].nth = nthreads;
     }
+    double vm[256];
+    kh_slab_vmax = vm;
     clock_gettime(CLOCK_MONOTONIC, &t0);
     for (int k = 1; k < nthreads; k++) pthread_create(&th[k], NULL, slab_worker, &args[k]);
     slab_worker(&args[0]);

@@ -144,3 +144,18 @@ def test_dba_conserved_totals_and_sleeping():
     for f in range(4):
         assert abs(V[f].sum() - U[f].sum()) <= 1e-12 * max(1.0, abs(U[f].sum()))
     assert o.awake(0).sum() < o.ntiles
+
+
+@pytest.mark.parametrize("walls,dt_mode", [(True, abi.DT_CFL), (False, abi.DT_FIXED)])
+def test_threaded_bench_equals_serial(walls, dt_mode):
+    """The CPU baseline's y-slab threads run the same operations as the serial step."""
+    p = configs.kh_params(n=96, tile=16, mode=abi.MODE_REGULAR, walls=walls, dt_mode=dt_mode)
+    U = configs.kh_ic(96)
+    a, b = OracleSim(p), OracleSim(p)
+    for f in range(4):
+        a.set_field(f, U[f])
+        b.set_field(f, U[f])
+    a.bench(6, 1)
+    b.bench(6, 5)
+    for f in range(4):
+        assert np.array_equal(a.get_field(f), b.get_field(f))
