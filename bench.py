@@ -54,27 +54,31 @@ WORKLOADS = {
 }
 
 
-def make(workload):
+def make(workload, n=None, tile=None, threads=0):
     from dba_amd import abi, configs
-    w = WORKLOADS[workload]
+    w = dict(WORKLOADS[workload])
+    if n:
+        w["n"] = n
+    if tile:
+        w["tile"] = tile
     mode = {"regular": abi.MODE_REGULAR, "dba": abi.MODE_DBA, "conserved": abi.MODE_CONSERVED}[w["mode"]]
     if w["model"] == "ch":
-        p = configs.ch_params(n=w["n"], tile=w["tile"], mode=mode)
+        p = configs.ch_params(n=w["n"], tile=w["tile"], mode=mode, threads=threads)
         fields = [configs.ch_ic(w["n"], w["n"])]
     elif w["model"] == "kh":
-        p = configs.kh_params(n=w["n"], tile=w["tile"], mode=mode)
+        p = configs.kh_params(n=w["n"], tile=w["tile"], mode=mode, threads=threads)
         fields = configs.kh_ic(w["n"])
     else:
-        p = configs.dendrite_k93_params(n=w["n"], tile=w["tile"], mode=mode)
+        p = configs.dendrite_k93_params(n=w["n"], tile=w["tile"], mode=mode, threads=threads)
         fields = list(configs.dendrite_k93_ic(w["n"], w["n"]))
     return p, fields
 
 
-def cpu_baseline(workload, budget_s=12.0):
+def cpu_baseline(workload, budget_s=12.0, n=None):
     """Oracle regular-mode loop on the host cores, bounded sample."""
     from dba_amd import abi
     from oracle.oracle import OracleSim
-    p, fields = make(workload)
+    p, fields = make(workload, n=n)
     p.mode = abi.MODE_REGULAR
     ncell = p.nx * p.ny
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
@@ -282,6 +286,9 @@ def main():
     ap.add_argument("--chunk", type=int, default=100, help="steps per captured hipGraph")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=500)
+    ap.add_argument("--n", type=int, default=None, help="grid side (ch, ch_cons, dendrite, kh); default: the workload's")
+    ap.add_argument("--tile", type=int, default=None, help="activation tile edge override")
+    ap.add_argument("--threads", type=int, default=0, help="workgroup size override (64/128/256)")
     ap.add_argument("--tune-steps", type=int, default=200, help="batch: timed steps per autotuner candidate")
     ap.add_argument("--requests", type=int, default=300, help="serve: requests in the stream")
     ap.add_argument("--rate", type=float, default=4.0, help="serve: initial arrival rate (1/s)")
@@ -306,7 +313,7 @@ def main():
     if args.workload == "copycal":
         print(json.dumps(copy_calibration()))
         return
-    p, fields = make(args.workload)
+    p, fields = make(args.workload, n=args.n, tile=args.tile, threads=args.threads)
     ncell = p.nx * p.ny
 
     with Context(0) as ctx:
@@ -360,8 +367,9 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"{args.workload}: {p.nx}x{p.ny} FP64, {w['tile']}x{w['tile']} tiles, {w['mode']}",
-            "grid": [p.nx, p.ny], "tile": w["tile"], "mode": w["mode"], "graph_chunk": chunk,
+            "workload": f"{args.workload}: {p.nx}x{p.ny} FP64, {p.tile_x}x{p.tile_y} tiles, {w['mode']}",
+            "grid": [p.nx, p.ny], "tile": p.tile_x, "threads": p.threads or "default", "mode": w["mode"],
+            "graph_chunk": chunk,
         },
         "full_grid_updates_per_s": K * ncell / wall_s,
         "active_fraction": active_frac,
@@ -373,7 +381,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_from_profile(args.workload),
+            "traffic": None if (args.n or args.tile or args.threads) else traffic_from_profile(args.workload),
             "kernel": "step kernel (fused stencil + detector)",
             "b_alg_per_cell": w["b_alg"],
             "avg_kernel_us": avg_k_s * 1e6,
@@ -382,7 +390,7 @@ def main():
         },
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.workload)
+        out["cpu_baseline"] = cpu_baseline(args.workload, n=args.n)
     print(json.dumps(out))
 
 

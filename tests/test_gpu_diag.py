@@ -17,9 +17,10 @@ def growth_front(phi, ix):
     """growth_front of example_Dendritic_growth/kernel.cu:760-774 (0 when not found)."""
     nz, ny, nx = phi.shape
     col = phi[nz // 2, :, ix]
-    for y in range(ny // 2, ny):
-        if col[y] < 0.5:
-            return (col[y - 1] - 0.5) / (col[y - 1] - col[y]) + y - 1 - ny // 2
+    with np.errstate(divide="ignore", invalid="ignore"):   # a flat column gives -inf, as on the device
+        for y in range(ny // 2, ny):
+            if col[y] < 0.5:
+                return (col[y - 1] - 0.5) / (col[y - 1] - col[y]) + y - 1 - ny // 2
     return 0.0
 
 
