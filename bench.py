@@ -183,6 +183,8 @@ def bench_batch(args, K):
         log(f"batched 16x16: {st16.wall_s:.3f} s")
         seq, _ = batch.run_sequential(ctx, items, K, tile=16, threads=0, chunk=args.chunk, fields_cache=cache)
         log(f"sequential: {seq.wall_s:.3f} s, {seq.active_rate:.4g} active/s")
+        plain, _ = batch.run_sequential(ctx, items, K, tile=16, threads=0, fields_cache=cache, plain=True)
+        log(f"sequential plain: {plain.wall_s:.3f} s, {plain.active_rate:.4g} active/s")
     out = {
         "metric": "active cell-updates/s of the batched 64-instance dendrite sweep (+ vs plain sequential DBA)",
         "value": st.active_rate, "unit": "cell-updates/s", "n_gpus": 1, "steps": K, "warmup": 0,
@@ -192,9 +194,13 @@ def bench_batch(args, K):
                    "tile": tile, "threads": nt, "graph_chunk": args.chunk, **batch.summary(items)},
         "full_grid_updates_per_s": st.full_rate,
         "active_fraction": st.active_updates / st.full_updates,
-        "batched_vs_sequential": st.active_rate / seq.active_rate,
-        "sequential": {"value": seq.active_rate, "wall_s": seq.wall_s, "tile": 16, "threads": "default",
-                       "note": "same library, one instance at a time, graph-replayed, stepping time only"},
+        "batched_vs_sequential_plain": st.active_rate / plain.active_rate,
+        "batched_vs_sequential_graph": st.active_rate / seq.active_rate,
+        "sequential_plain": {"value": plain.active_rate, "wall_s": plain.wall_s, "tile": 16, "threads": "default",
+                             "note": "one instance at a time, one step launched and waited for per time step "
+                                     "(the reference loop's shape), stepping time only"},
+        "sequential_graph": {"value": seq.active_rate, "wall_s": seq.wall_s, "tile": 16, "threads": "default",
+                             "note": "one instance at a time, graph-replayed chunks of steps, stepping time only"},
         "batched_16x16": {"value": st16.active_rate, "wall_s": st16.wall_s,
                           "vs_sequential_wall": seq.wall_s / st16.wall_s},
         "autotune": [{"tile": a, "threads": b, "ms_per_step": c, "active_rate": d} for a, b, c, d in table],
@@ -222,7 +228,7 @@ def bench_batch(args, K):
 
 def bench_serve(args):
     """Continuous-batching server on a Poisson stream; sequential sample alone."""
-    from dba_amd import Context, serve
+    from dba_amd import Context, batch, serve
     log = (lambda m: print(m, file=sys.stderr, flush=True))
     rs = serve.make_requests(args.requests)
     tiles = {"dendrite": 16, "ch": 32}
@@ -235,26 +241,31 @@ def bench_serve(args):
         rep = serve.report(done, srv.wall, srv.occupancy)
         log(f"served {rep['completed']} in {srv.wall:.1f} s, p50 {rep['p50_latency_s']:.3f} s, "
             f"p99 {rep['p99_latency_s']:.3f} s, {rep['active_rate']:.4g} active/s")
-        # plain sequential loop: requests alone, in stream order, until the time budget
-        seq_t = seq_u = 0.0
-        nseq = 0
-        for r in rs:
-            if seq_t > args.seq_budget_s:
-                break
-            inst = ctx.create(r.params(tiles, {}))
-            for f, a in enumerate(r.fields()):
-                inst.upload(f, a)
-            ctx.sync()
-            u0 = inst.info().cell_updates
-            t0 = time.perf_counter()
-            inst.step(r.steps, graph_chunk=args.chunk)
-            ctx.sync()
-            seq_t += time.perf_counter() - t0
-            seq_u += inst.info().cell_updates - u0
-            nseq += 1
-            inst.destroy()
-        seq_rate = seq_u / seq_t
-        log(f"sequential sample: {nseq} requests, {seq_rate:.4g} active/s")
+        # sequential loops: the first requests of the stream alone, until the time budget
+        seq = {}
+        for mode in ("graph", "plain"):
+            seq_t = seq_u = 0.0
+            nseq = 0
+            for r in rs:
+                if seq_t > args.seq_budget_s:
+                    break
+                inst = ctx.create(r.params(tiles, {}))
+                for f, a in enumerate(r.fields()):
+                    inst.upload(f, a)
+                ctx.sync()
+                u0 = inst.info().cell_updates
+                t0 = time.perf_counter()
+                if mode == "graph":
+                    inst.step(r.steps, graph_chunk=args.chunk)
+                else:
+                    batch.step_plain(ctx, inst, r.steps)
+                ctx.sync()
+                seq_t += time.perf_counter() - t0
+                seq_u += inst.info().cell_updates - u0
+                nseq += 1
+                inst.destroy()
+            seq[mode] = (seq_u / seq_t, nseq, seq_t)
+            log(f"sequential {mode} sample: {nseq} requests, {seq[mode][0]:.4g} active/s")
     lat = sorted(c.latency for c in done)
     out = {
         "metric": "aggregate active cell-updates/s of the served dendrite + CH stream (+ p50/p99 latency)",
@@ -268,9 +279,13 @@ def bench_serve(args):
         **rep,
         "latency_s": {"p50": rep["p50_latency_s"], "p90": serve.percentile(lat, 90), "p99": rep["p99_latency_s"],
                       "max": lat[-1]},
-        "served_vs_sequential": rep["active_rate"] / seq_rate,
-        "sequential": {"value": seq_rate, "requests": nseq, "wall_s": seq_t,
-                       "note": "the first requests of the same stream run alone, one at a time"},
+        "served_vs_sequential_plain": rep["active_rate"] / seq["plain"][0],
+        "served_vs_sequential_graph": rep["active_rate"] / seq["graph"][0],
+        "sequential_plain": {"value": seq["plain"][0], "requests": seq["plain"][1], "wall_s": seq["plain"][2],
+                             "note": "the first requests of the same stream run alone, one step launched and "
+                                     "waited for per time step (the reference loop's shape)"},
+        "sequential_graph": {"value": seq["graph"][0], "requests": seq["graph"][1], "wall_s": seq["graph"][2],
+                             "note": "the same requests alone, graph-replayed chunks of steps"},
         "ticks": srv.ticks, "max_used_bytes": srv.max_used,
         "final_rate_per_s": srv.rate_trace[-1][1] if srv.rate_trace else args.rate,
     }

@@ -3,10 +3,11 @@
 All instances of one model, tile and workgroup size form one group in the
 library: each step is ONE stencil launch over the merged cross-instance
 worklist (ascending tile order per instance, instances in call order) plus ONE
-finalize launch, replayed from a captured hipGraph. The plain alternative runs
-one instance at a time through the same library (two launches per instance per
-step), which is the reference's one-simulation-per-process loop
-(example_Dendritic_growth/kernel.cu:913-1011) without its host syncs.
+finalize launch, replayed from a captured hipGraph. The alternatives run one
+instance at a time through the same library: `plain` launches one step and
+waits for it per time step, the shape of the reference's
+one-simulation-per-process loop (example_Dendritic_growth/kernel.cu:913-1011);
+`graph` replays captured chunks of steps.
 
 The sweep (64 instances by default):
   side      log-uniform in [256, 2048], rounded to a multiple of 32 (so tiles of
@@ -124,8 +125,18 @@ def run_batched(ctx, items, steps, tile=16, threads=0, chunk=100, fields_cache=N
     return st, None
 
 
-def run_sequential(ctx, items, steps, tile=16, threads=0, chunk=100, fields_cache=None, keep=False):
-    """One instance at a time (create, upload, step, next); stepping time only."""
+def step_plain(ctx, inst, steps):
+    """The reference's loop shape: one step launched, then waited for, per time
+    step (example_Dendritic_growth/kernel.cu:913-1011 synchronises after every
+    kernel); no graph."""
+    for _ in range(steps):
+        inst.step(1)
+        ctx.sync()
+
+
+def run_sequential(ctx, items, steps, tile=16, threads=0, chunk=100, fields_cache=None, keep=False, plain=False):
+    """One instance at a time (create, upload, step, next); stepping time only.
+    plain=True steps like the reference's loop (step_plain) instead of replaying graphs."""
     wall = gpu = 0.0
     upd = 0
     kept = []
@@ -135,7 +146,10 @@ def run_sequential(ctx, items, steps, tile=16, threads=0, chunk=100, fields_cach
         u0 = inst.info().cell_updates
         ctx.marker(0)
         t0 = time.perf_counter()
-        inst.step(steps, graph_chunk=min(chunk, steps) if steps else 0)
+        if plain:
+            step_plain(ctx, inst, steps)
+        else:
+            inst.step(steps, graph_chunk=min(chunk, steps) if steps else 0)
         ctx.marker(1)
         ctx.sync()
         wall += time.perf_counter() - t0

@@ -24,6 +24,7 @@ struct Inst {
     int64_t* ctr;                // [0] steps done, [1] active cell updates, [2] worklist length
     int32_t* hist;               // per-step active tile counts, [2*step + set]
     int32_t* seg;                // this instance's compacted worklist (ascending tiles)
+    uint64_t* nbrec;             // per worklist position (2D): the 3x3 neighbours' parity and awake bits
     int32_t hist_cap;
     int32_t model, mode;
     int32_t nx, ny, nz;
@@ -129,21 +130,44 @@ struct NbTable {
     double shift[kMaxFields][9];
 };
 
-// Fill by threads 0..8; caller synchronises.
+// Owner of the halo cells of tile t (2D, edge tb) in direction d = (dx+1) + 3 (dy+1):
+// the tile holding the cell just outside the tile's edge on that side after the
+// boundary map (wrap, or mirror back into t).
+__device__ __forceinline__ int nb_owner2d(const Inst& I, int t, int tb, int d) {
+    const int x0 = (t % I.ntx) * tb, y0 = (t / I.ntx) * tb;
+    const int dx = d % 3 - 1, dy = d / 3 - 1;
+    const int rx = dx < 0 ? x0 - 1 : (dx > 0 ? x0 + tb : x0);
+    const int ry = dy < 0 ? y0 - 1 : (dy > 0 ? y0 + tb : y0);
+    const int mx = map_index(rx, I.nx, I.bc_x), my = map_index(ry, I.ny, I.bc_y == 0 ? 0 : 1);
+    return mx / tb + (my / tb) * I.ntx;
+}
+
+// Neighbour record written by the finalize kernel next to each worklist entry
+// (2D models): bit s*18 + d = buffer parity, bit s*18 + 9 + d = awake, of the
+// owner tile in direction d for set s, as the step reading it will see them;
+// kRecValid marks a record whose nine owners all lie in the finalizing segment.
+constexpr uint64_t kRecValid = 1ull << 36;
+
+// Fill by threads 0..8; caller synchronises. With a valid record the parity and
+// awake bits come from it; otherwise they are read from the bitmaps.
 template <int TB>
 __device__ __forceinline__ void nb_fill(NbTable& nb, const Inst& I, int t, int x0, int y0, bool regular,
-                                        int nsets, int ncons) {
+                                        int nsets, int ncons, uint64_t rec = 0) {
+    (void)x0;
+    (void)y0;
     const int d = threadIdx.x;
     if (d >= 9) return;
-    const int dx = d % 3 - 1, dy = d / 3 - 1;
-    const int rx = dx < 0 ? x0 - 1 : (dx > 0 ? x0 + TB : x0);
-    const int ry = dy < 0 ? y0 - 1 : (dy > 0 ? y0 + TB : y0);
-    const int mx = map_index(rx, I.nx, I.bc_x), my = map_index(ry, I.ny, I.bc_y == 0 ? 0 : 1);
-    const int tt = mx / TB + (my / TB) * I.ntx;
+    const int tt = nb_owner2d(I, t, TB, d);
     nb.tile[d] = tt;
+    const bool have = (rec & kRecValid) != 0;
     for (int s = 0; s < nsets; s++) {
-        nb.par[s][d] = bit_of(I.par[s], tt);
-        nb.act[s][d] = regular ? 1 : bit_of(I.a_cur[s], tt);
+        if (have) {
+            nb.par[s][d] = (int)((rec >> (s * 18 + d)) & 1ull);
+            nb.act[s][d] = (int)((rec >> (s * 18 + 9 + d)) & 1ull);
+        } else {
+            nb.par[s][d] = bit_of(I.par[s], tt);
+            nb.act[s][d] = regular ? 1 : bit_of(I.a_cur[s], tt);
+        }
     }
     for (int f = 0; f < kMaxFields; f++)
         nb.shift[f][d] = f < ncons ? (I.scal[8 + f] - I.s_tile[((int64_t)f * 2 + nb.par[0][d]) * I.nt + tt]) : 0.0;
@@ -236,25 +260,33 @@ struct WorkList {
     const int32_t* offsets;   // exclusive prefix of segment counts, [nseg] = total
     int nseg;
     const int32_t* solo_seg;  // single-instance single-segment groups: its list, else null
+    const uint64_t* solo_rec; // ... its neighbour records
     int solo_nt;              // ... and its tile count (0 otherwise)
 };
 
-// Item -> (instance, tile); false once the item is past the list. `total` is
-// offsets[nseg], loaded by the caller once per workgroup.
-__device__ __forceinline__ bool next_item(const WorkList& w, int item, int total, const Inst** I, int* t) {
+// Item -> (instance, tile, neighbour record); false once the item is past the
+// list. `total` is offsets[nseg], loaded by the caller once per workgroup.
+// 3D models pass want_rec = false (their records are not written).
+__device__ __forceinline__ bool next_item(const WorkList& w, int item, int total, const Inst** I, int* t,
+                                          uint64_t* rec, bool want_rec = true) {
     if (w.solo_nt > 0) {
         if (item >= w.solo_nt) return false;
-        const int tt = w.solo_seg[item];  // in bounds; loaded before the length check so both loads overlap
+        // in bounds; loaded before the length check so all three loads overlap
+        const int tt = w.solo_seg[item];
+        const uint64_t r = want_rec ? w.solo_rec[item] : 0ull;
         if (item >= total) return false;
         *I = w.insts;
         *t = tt;
+        *rec = r;
         return true;
     }
     if (item >= total) return false;
     const int s = w.nseg == 1 ? 0 : find_slot(w.offsets, w.nseg, item);
     const Seg sg = w.segs[s];
     *I = w.insts + sg.inst;
-    *t = (*I)->seg[sg.tile0 + (item - w.offsets[s])];
+    const int pos = sg.tile0 + (item - w.offsets[s]);
+    *t = (*I)->seg[pos];
+    *rec = want_rec ? (*I)->nbrec[pos] : 0ull;
     return true;
 }
 

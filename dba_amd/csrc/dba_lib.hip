@@ -101,6 +101,7 @@ struct Group {
     int grid = 0;
     uint64_t last_use = 0;
     int32_t* solo_seg = nullptr;  // one instance in one segment: its worklist ...
+    uint64_t* solo_rec = nullptr; // ... its neighbour records
     int solo_nt = 0;              // ... and tile count
 };
 
@@ -243,6 +244,7 @@ int64_t footprint_of(const dba_params& p) {
     b += 16 * 8 + 4 * 8;                     // scal + ctr
     b += (int64_t)2 * std::max(0, p.history_cap) * 4;
     b += (int64_t)L.nt * 4;                  // seg
+    b += (int64_t)L.nt * 8;                  // nbrec
     b += sizeof(Inst);
     return b;
 }
@@ -392,6 +394,7 @@ dba_status get_group(dba_ctx* ctx, const std::vector<int>& ids, Group** out) {
     Instance* first = ctx->inst[ids[0]];
     if (ids.size() == 1 && g->nseg == 1) {
         g->solo_seg = first->h.seg;
+        g->solo_rec = first->h.nbrec;
         g->solo_nt = first->h.nt;
     }
     g->model = first->p.model;
@@ -425,7 +428,7 @@ dba_status get_group(dba_ctx* ctx, const std::vector<int>& ids, Group** out) {
 // ---------------------------------------------------------------- launches
 
 WorkList wl_of(const Group* g) {
-    return WorkList{g->d_insts, g->d_segs, g->d_offsets, g->nseg, g->solo_seg, g->solo_nt};
+    return WorkList{g->d_insts, g->d_segs, g->d_offsets, g->nseg, g->solo_seg, g->solo_rec, g->solo_nt};
 }
 
 template <int TB, int NT>
@@ -740,6 +743,7 @@ dba_status dba_instance_create(dba_ctx* ctx, const dba_params* pp, int32_t* id) 
     ALLOC(h.ctr, 4);
     ALLOC(h.hist, (size_t)2 * h.hist_cap + 2);
     ALLOC(h.seg, (size_t)L.nt);
+    ALLOC(h.nbrec, (size_t)L.nt);
     ALLOC(in->d_self, 1);
 #undef ALLOC
     // all tiles awake at start (example_Dendritic_growth/kernel.cu:149-161)

@@ -50,8 +50,8 @@ __device__ __forceinline__ void strip_lap(const double* s, int cx, int cy0, doub
 }
 
 template <int TB, int NT, bool NINE>
-__device__ __forceinline__ void ch_tile(const Inst& I, int t, NbTable& nb, double* sc, double* smu, double* sred,
-                                        int* sflag) {
+__device__ __forceinline__ void ch_tile(const Inst& I, int t, uint64_t rec, NbTable& nb, double* sc, double* smu,
+                                        double* sred, int* sflag) {
     constexpr int W2 = TB + 4;
     constexpr int W1 = TB + 2;
     constexpr int R = TB * TB / NT;
@@ -68,7 +68,7 @@ __device__ __forceinline__ void ch_tile(const Inst& I, int t, NbTable& nb, doubl
     double* __restrict__ mu_store = I.aux[0];
     const double kappa = I.kappa, eps = I.thresh_a;
     const double inv = NINE ? I.inv6dx2 : I.invdx2;
-    nb_fill<TB>(nb, I, t, x0, y0, regular, 1, cons ? 1 : 0);
+    nb_fill<TB>(nb, I, t, x0, y0, regular, 1, cons ? 1 : 0, rec);
     __syncthreads();
     const int pself = nb.par[0][4];
     const double sh_self = nb.shift[0][4];
@@ -174,10 +174,11 @@ __global__ __launch_bounds__(NT) void ch_step_kernel(const WorkList wl) {
     for (int item = xcd_remap(blockIdx.x, G);; item += G) {
         const Inst* Ip;
         int t;
-        if (!next_item(wl, item, total, &Ip, &t)) break;
+        uint64_t rec;
+        if (!next_item(wl, item, total, &Ip, &t, &rec)) break;
         const Inst& I = *Ip;
-        if (I.stencil != 5) ch_tile<TB, NT, true>(I, t, nb, sc, smu, sred, sflag);
-        else ch_tile<TB, NT, false>(I, t, nb, sc, smu, sred, sflag);
+        if (I.stencil != 5) ch_tile<TB, NT, true>(I, t, rec, nb, sc, smu, sred, sflag);
+        else ch_tile<TB, NT, false>(I, t, rec, nb, sc, smu, sred, sflag);
     }
 }
 

@@ -149,6 +149,7 @@ __device__ __forceinline__ void finish_instance(const Inst& J, int64_t listlen, 
 // LDS of the finalize body.
 struct FinLds {
     uint64_t s_cur[2][kSegWords];
+    uint64_t s_par[2][kSegWords];   // buffer parity words the next step sees
     uint64_t s_new[2][kSegWords];
     int s_pre[kSegWords + 1];
     double sd[5];
@@ -172,6 +173,7 @@ __device__ __forceinline__ void finalize_body(const Inst* __restrict__ insts, co
     constexpr int WPR = NT / 64;        // words per round
     auto& s_cur = F.s_cur;
     auto& s_new = F.s_new;
+    auto& s_par = F.s_par;
     auto& s_pre = F.s_pre;
     auto& sd = F.sd;
     auto& si = F.si;
@@ -200,7 +202,7 @@ __device__ __forceinline__ void finalize_body(const Inst* __restrict__ insts, co
     static_assert(2 * kSegWords <= 64, "word threads fit in one wave");
     if (wthread) {
         cur = regular ? full_word(w0 + wj, nt) : I.a_cur[ws][w0 + wj];
-        if (rotate) parw = I.par[ws][w0 + wj];
+        parw = I.par[ws][w0 + wj];
     }
     int a0[KT], a1[KT];
     {
@@ -228,8 +230,13 @@ __device__ __forceinline__ void finalize_body(const Inst* __restrict__ insts, co
     if (solo && tid == 16) s_step = I.ctr[0];
     if (solo && tid == 17) s_upd = I.ctr[1];
     if (tid < 3) s_cnt[tid] = 0ull;
-    if (wthread) s_cur[ws][wj] = cur;
-    else if (tid < 2 * kSegWords) s_cur[ws][wj] = 0ull;
+    if (wthread) {
+        s_cur[ws][wj] = cur;
+        s_par[ws][wj] = rotate ? (parw ^ cur) : parw;
+    } else if (tid < 2 * kSegWords) {
+        s_cur[ws][wj] = 0ull;
+        s_par[ws][wj] = 0ull;
+    }
     __syncthreads();
 
     if (rotate) {
@@ -307,13 +314,32 @@ __device__ __forceinline__ void finalize_body(const Inst* __restrict__ insts, co
         if (tid == kSegWords - 1) s_pre[kSegWords] = incl;
     }
     __syncthreads();
+    const bool flat2 = I.ntz == 1;
 #pragma unroll
     for (int i = 0; i < KT; i++) {
         const int k = i * NT + tid;
         if (k >= nts) break;
         const int j = k >> 6, b = k & 63;
         const uint64_t u = s_new[0][j] | s_new[1][j];
-        if ((u >> b) & 1ull) I.seg[t0 + s_pre[j] + __popcll(u & ((1ull << b) - 1ull))] = t0 + k;
+        if (!((u >> b) & 1ull)) continue;
+        const int pos = t0 + s_pre[j] + __popcll(u & ((1ull << b) - 1ull));
+        I.seg[pos] = t0 + k;
+        if (flat2) {
+            // neighbour record: resolved here when all nine owners are in this segment
+            uint64_t rec = kRecValid;
+            for (int d = 0; d < 9; d++) {
+                const int o = nb_owner2d(I, t0 + k, I.tx, d) - t0;
+                if (o < 0 || o >= nts) {
+                    rec = 0ull;
+                    break;
+                }
+                for (int s2 = 0; s2 < nsets; s2++) {
+                    rec |= ((s_par[s2][o >> 6] >> (o & 63)) & 1ull) << (s2 * 18 + d);
+                    rec |= ((s_new[s2][o >> 6] >> (o & 63)) & 1ull) << (s2 * 18 + 9 + d);
+                }
+            }
+            I.nbrec[pos] = rec;
+        }
     }
 
     // (d) per-instance reductions and merged offsets

@@ -59,7 +59,8 @@ __global__ __launch_bounds__(NT) void k93_step_kernel(const WorkList wl) {
     for (int item = xcd_remap(blockIdx.x, G);; item += G) {
         const Inst* Ip;
         int t;
-        if (!next_item(wl, item, total, &Ip, &t)) break;
+        uint64_t rec;
+        if (!next_item(wl, item, total, &Ip, &t, &rec)) break;
         const Inst& I = *Ip;
         const int nx = I.nx, ny = I.ny, ntx = I.ntx, bcx = I.bc_x, bcy = I.bc_y;
         const int x0 = (t % ntx) * TB, y0 = (t / ntx) * TB;
@@ -68,7 +69,7 @@ __global__ __launch_bounds__(NT) void k93_step_kernel(const WorkList wl) {
         const double* __restrict__ p1 = I.buf[1][0];
         const double* __restrict__ T0 = I.buf[0][1];
         const double* __restrict__ T1 = I.buf[1][1];
-        nb_fill<TB>(nb, I, t, x0, y0, regular, 2, 0);
+        nb_fill<TB>(nb, I, t, x0, y0, regular, 2, 0, rec);
         __syncthreads();
         const int ap = nb.act[0][4], at = nb.act[1][4];
         const int pp0 = nb.par[0][4], pp1 = nb.par[1][4];

@@ -90,7 +90,8 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
     for (int item = xcd_remap(blockIdx.x, G);; item += G) {
         const Inst* Ip;
         int t;
-        if (!next_item(wl, item, total, &Ip, &t)) break;
+        uint64_t rec;
+        if (!next_item(wl, item, total, &Ip, &t, &rec)) break;
         const Inst& I = *Ip;
         const int nx = I.nx, ny = I.ny, ntx = I.ntx;
         const int x0 = (t % ntx) * TB, y0 = (t / ntx) * TB;
@@ -105,7 +106,7 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
         c.dy = I.dy;
         c.vol = I.vol;
         c.dt = cfl ? I.scal[1] : I.dt;
-        nb_fill<TB>(nb, I, t, x0, y0, regular, 1, ncons);
+        nb_fill<TB>(nb, I, t, x0, y0, regular, 1, ncons, rec);
         __syncthreads();
         const int pself = nb.par[0][4];
 

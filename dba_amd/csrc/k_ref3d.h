@@ -92,7 +92,8 @@ __global__ __launch_bounds__(64) void modelb_step_kernel(const WorkList wl) {
     for (int item = xcd_remap(blockIdx.x, G);; item += G) {
         const Inst* Ip;
         int t;
-        if (!next_item(wl, item, total, &Ip, &t)) break;
+        uint64_t rec;
+        if (!next_item(wl, item, total, &Ip, &t, &rec, false)) break;
         const Inst& I = *Ip;
         const int nx = I.nx, ny = I.ny, nz = I.nz;
         const int x0 = (t % I.ntx) * TB, y0 = ((t / I.ntx) % I.nty) * TB, z0 = (t / (I.ntx * I.nty)) * TZ;
@@ -211,7 +212,8 @@ __global__ __launch_bounds__(64) void k94_step_kernel(const WorkList wl) {
     for (int item = xcd_remap(blockIdx.x, G);; item += G) {
         const Inst* Ip;
         int t;
-        if (!next_item(wl, item, total, &Ip, &t)) break;
+        uint64_t rec;
+        if (!next_item(wl, item, total, &Ip, &t, &rec, false)) break;
         const Inst& I = *Ip;
         const int nx = I.nx, ny = I.ny, nz = I.nz;
         const int x0 = (t % I.ntx) * TB, y0 = ((t / I.ntx) % I.nty) * TB, z0 = (t / (I.ntx * I.nty)) * TZ;
