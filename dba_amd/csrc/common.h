@@ -6,25 +6,43 @@
 
 namespace dba {
 
+// Address spaces for device code: the descriptor's data pointers are global
+// memory (so accesses compile to global_load/global_store rather than flat
+// instructions, which also count against the LDS wait counter), and the
+// descriptor itself is read through the constant address space (scalar loads).
+// In the host pass both are plain, so the layout is the same on both sides.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define DBA_GLOBAL __attribute__((address_space(1)))
+#define DBA_CONST __attribute__((address_space(4)))
+#else
+#define DBA_GLOBAL
+#define DBA_CONST
+#endif
+typedef DBA_GLOBAL double gdouble;
+typedef DBA_GLOBAL uint64_t gu64;
+typedef DBA_GLOBAL uint8_t gu8;
+typedef DBA_GLOBAL int32_t gi32;
+typedef DBA_GLOBAL int64_t gi64;
+
 constexpr int kMaxFields = 4;
 constexpr int kMaxAux = 32;
 
 // One simulation instance as the kernels see it. Lives in device memory
 // (an array of these per merged group); everything the step needs is here.
 struct Inst {
-    double* buf[2][kMaxFields];  // double-buffered prognostic fields (tile parity picks one)
-    double* aux[kMaxAux];        // model state kept for sleeping-neighbour semantics
-    uint64_t* a_cur[2];          // awake bitmap of the step being executed, per set
-    uint8_t* nflag[2];           // detector output per tile (Wakeup_Next), per set
-    uint64_t* par[2];            // per-tile buffer parity per set: bit=1 -> buf[1] is current
-    double* tpart;               // conserved: per-tile change sums, [field][nt]
-    double* tspeed;              // KH: per-tile max wave speed, [0..nt) fresh (current U), [nt..2nt) stale
-    double* s_tile;              // conserved: per-tile epoch of the cumulative shift, [field][parity][nt]
-    double* scal;                // [1] dt of the next step, [2] time, [3] last dt, [8+f] cumulative shift S_f
-    int64_t* ctr;                // [0] steps done, [1] active cell updates, [2] worklist length
-    int32_t* hist;               // per-step active tile counts, [2*step + set]
-    int32_t* seg;                // this instance's compacted worklist (ascending tiles)
-    uint64_t* nbrec;             // per worklist position (2D): the 3x3 neighbours' parity and awake bits
+    gdouble* buf[2][kMaxFields];  // double-buffered prognostic fields (tile parity picks one)
+    gdouble* aux[kMaxAux];        // model state kept for sleeping-neighbour semantics
+    gu64* a_cur[2];               // awake bitmap of the step being executed, per set
+    gu8* nflag[2];                // detector output per tile (Wakeup_Next), per set
+    gu64* par[2];                 // per-tile buffer parity per set: bit=1 -> buf[1] is current
+    gdouble* tpart;               // conserved: per-tile change sums, [field][nt]
+    gdouble* tspeed;              // KH: per-tile max wave speed, [0..nt) fresh (current U), [nt..2nt) stale
+    gdouble* s_tile;              // conserved: per-tile epoch of the cumulative shift, [field][parity][nt]
+    gdouble* scal;                // [1] dt of the next step, [2] time, [3] last dt, [8+f] cumulative shift S_f
+    gi64* ctr;                    // [0] steps done, [1] active cell updates, [2] worklist length
+    gi32* hist;                   // per-step active tile counts, [2*step + set]
+    gi32* seg;                    // this instance's compacted worklist (ascending tiles)
+    gu64* nbrec;                  // per worklist position (2D): the 3x3 neighbours' parity and awake bits
     int32_t hist_cap;
     int32_t model, mode;
     int32_t nx, ny, nz;
@@ -53,6 +71,9 @@ struct Inst {
     double gas_gamma, cfl, vol;
 };
 
+// The descriptor as device code reads it (scalar loads).
+using CInst = const DBA_CONST Inst;
+
 // Boundary index map: periodic wrap, or mirror about the boundary face.
 __device__ __forceinline__ int map_index(int i, int n, int bc) {
     if (bc == 0) {
@@ -65,7 +86,7 @@ __device__ __forceinline__ int map_index(int i, int n, int bc) {
     return i;
 }
 
-__device__ __forceinline__ int bit_of(const uint64_t* bm, int t) {
+__device__ __forceinline__ int bit_of(const gu64* bm, int t) {
     return (int)((bm[t >> 6] >> (t & 63)) & 1ull);
 }
 
@@ -130,16 +151,18 @@ struct NbTable {
     double shift[kMaxFields][9];
 };
 
-// Owner of the halo cells of tile t (2D, edge tb) in direction d = (dx+1) + 3 (dy+1):
-// the tile holding the cell just outside the tile's edge on that side after the
-// boundary map (wrap, or mirror back into t).
-__device__ __forceinline__ int nb_owner2d(const Inst& I, int t, int tb, int d) {
-    const int x0 = (t % I.ntx) * tb, y0 = (t / I.ntx) * tb;
-    const int dx = d % 3 - 1, dy = d / 3 - 1;
-    const int rx = dx < 0 ? x0 - 1 : (dx > 0 ? x0 + tb : x0);
-    const int ry = dy < 0 ? y0 - 1 : (dy > 0 ? y0 + tb : y0);
-    const int mx = map_index(rx, I.nx, I.bc_x), my = map_index(ry, I.ny, I.bc_y == 0 ? 0 : 1);
-    return mx / tb + (my / tb) * I.ntx;
+// Owner of the halo cells of tile (ti, tj) (2D) in direction d = (dx+1) + 3 (dy+1):
+// the tile holding the cells just outside the tile's edge on that side after
+// the boundary map: the neighbour, wrapped when periodic, the tile itself when
+// the boundary mirrors (zero-flux or wall).
+__device__ __forceinline__ int nb_step1(int i, int di, int n, int bc) {
+    if (di < 0) return i > 0 ? i - 1 : (bc == 0 ? n - 1 : i);
+    if (di > 0) return i < n - 1 ? i + 1 : (bc == 0 ? 0 : i);
+    return i;
+}
+
+__device__ __forceinline__ int nb_owner2d(CInst& I, int ti, int tj, int d) {
+    return nb_step1(ti, d % 3 - 1, I.ntx, I.bc_x) + nb_step1(tj, d / 3 - 1, I.nty, I.bc_y == 0 ? 0 : 1) * I.ntx;
 }
 
 // Neighbour record written by the finalize kernel next to each worklist entry
@@ -151,13 +174,14 @@ constexpr uint64_t kRecValid = 1ull << 36;
 // Fill by threads 0..8; caller synchronises. With a valid record the parity and
 // awake bits come from it; otherwise they are read from the bitmaps.
 template <int TB>
-__device__ __forceinline__ void nb_fill(NbTable& nb, const Inst& I, int t, int x0, int y0, bool regular,
+__device__ __forceinline__ void nb_fill(NbTable& nb, CInst& I, int t, int x0, int y0, bool regular,
                                         int nsets, int ncons, uint64_t rec = 0) {
     (void)x0;
     (void)y0;
     const int d = threadIdx.x;
     if (d >= 9) return;
-    const int tt = nb_owner2d(I, t, TB, d);
+    const int tj0 = t / I.ntx;
+    const int tt = nb_owner2d(I, t - tj0 * I.ntx, tj0, d);
     nb.tile[d] = tt;
     const bool have = (rec & kRecValid) != 0;
     for (int s = 0; s < nsets; s++) {
@@ -238,7 +262,7 @@ __device__ __forceinline__ int find_slot(const int32_t* offsets, int n, int item
 
 // Current value of a field at cell i: the buffer the tile parity selects,
 // minus the conserved shift accumulated since the tile last ran.
-__device__ __forceinline__ double current_value(const Inst& I, int field, int64_t i) {
+__device__ __forceinline__ double current_value(CInst& I, int field, int64_t i) {
     const int x = (int)(i % I.nx);
     const int y = (int)((i / I.nx) % I.ny);
     const int z = (int)(i / ((int64_t)I.nx * I.ny));
@@ -267,7 +291,7 @@ struct WorkList {
 // Item -> (instance, tile, neighbour record); false once the item is past the
 // list. `total` is offsets[nseg], loaded by the caller once per workgroup.
 // 3D models pass want_rec = false (their records are not written).
-__device__ __forceinline__ bool next_item(const WorkList& w, int item, int total, const Inst** I, int* t,
+__device__ __forceinline__ bool next_item(const WorkList& w, int item, int total, CInst** I, int* t,
                                           uint64_t* rec, bool want_rec = true) {
     if (w.solo_nt > 0) {
         if (item >= w.solo_nt) return false;
@@ -275,18 +299,21 @@ __device__ __forceinline__ bool next_item(const WorkList& w, int item, int total
         const int tt = w.solo_seg[item];
         const uint64_t r = want_rec ? w.solo_rec[item] : 0ull;
         if (item >= total) return false;
-        *I = w.insts;
-        *t = tt;
-        *rec = r;
+        *I = (CInst*)w.insts;
+        *t = __builtin_amdgcn_readfirstlane(tt);
+        *rec = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)r)) |
+               ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(r >> 32)) << 32);
         return true;
     }
     if (item >= total) return false;
     const int s = w.nseg == 1 ? 0 : find_slot(w.offsets, w.nseg, item);
     const Seg sg = w.segs[s];
-    *I = w.insts + sg.inst;
-    const int pos = sg.tile0 + (item - w.offsets[s]);
-    *t = (*I)->seg[pos];
-    *rec = want_rec ? (*I)->nbrec[pos] : 0ull;
+    *I = (CInst*)w.insts + __builtin_amdgcn_readfirstlane(sg.inst);
+    const int pos = __builtin_amdgcn_readfirstlane(sg.tile0 + (item - w.offsets[s]));
+    *t = __builtin_amdgcn_readfirstlane((*I)->seg[pos]);
+    const uint64_t r = want_rec ? (*I)->nbrec[pos] : 0ull;
+    *rec = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)r)) |
+           ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(r >> 32)) << 32);
     return true;
 }
 

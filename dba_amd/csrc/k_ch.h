@@ -50,7 +50,7 @@ __device__ __forceinline__ void strip_lap(const double* s, int cx, int cy0, doub
 }
 
 template <int TB, int NT, bool NINE>
-__device__ __forceinline__ void ch_tile(const Inst& I, int t, uint64_t rec, NbTable& nb, double* sc, double* smu,
+__device__ __forceinline__ void ch_tile(CInst& I, int t, uint64_t rec, NbTable& nb, double* sc, double* smu,
                                         double* sred, int* sflag) {
     constexpr int W2 = TB + 4;
     constexpr int W1 = TB + 2;
@@ -63,17 +63,17 @@ __device__ __forceinline__ void ch_tile(const Inst& I, int t, uint64_t rec, NbTa
     const int x0 = (t % ntx) * TB, y0 = (t / ntx) * TB;
     const bool regular = I.mode == 0;
     const bool cons = I.mode == 2;
-    const double* __restrict__ b0 = I.buf[0][0];
-    const double* __restrict__ b1 = I.buf[1][0];
-    double* __restrict__ mu_store = I.aux[0];
+    const gdouble* __restrict__ b0 = I.buf[0][0];
+    const gdouble* __restrict__ b1 = I.buf[1][0];
+    gdouble* __restrict__ mu_store = I.aux[0];
     const double kappa = I.kappa, eps = I.thresh_a;
     const double inv = NINE ? I.inv6dx2 : I.invdx2;
     nb_fill<TB>(nb, I, t, x0, y0, regular, 1, cons ? 1 : 0, rec);
     __syncthreads();
     const int pself = nb.par[0][4];
     const double sh_self = nb.shift[0][4];
-    const double* __restrict__ cur = pself ? b1 : b0;
-    double* __restrict__ nxt = pself ? I.buf[0][0] : I.buf[1][0];
+    const gdouble* __restrict__ cur = pself ? b1 : b0;
+    gdouble* __restrict__ nxt = pself ? I.buf[0][0] : I.buf[1][0];
 
     // 1. c: the thread's strip from HBM into registers and LDS, then the halo
     double c[R];
@@ -172,11 +172,11 @@ __global__ __launch_bounds__(NT) void ch_step_kernel(const WorkList wl) {
     const int total = wl.offsets[wl.nseg];
     const int G = gridDim.x;
     for (int item = xcd_remap(blockIdx.x, G);; item += G) {
-        const Inst* Ip;
+        CInst* Ip;
         int t;
         uint64_t rec;
         if (!next_item(wl, item, total, &Ip, &t, &rec)) break;
-        const Inst& I = *Ip;
+        CInst& I = *Ip;
         if (I.stencil != 5) ch_tile<TB, NT, true>(I, t, rec, nb, sc, smu, sred, sflag);
         else ch_tile<TB, NT, false>(I, t, rec, nb, sc, smu, sred, sflag);
     }

@@ -24,7 +24,7 @@ constexpr int kMaxFrontCols = 64;
 __global__ __launch_bounds__(kDiagNT) void field_sum_kernel(const Inst* __restrict__ insts, int slot, int field,
                                                             double* __restrict__ part) {
     __shared__ double sred[kDiagNT / 64 + 1];
-    const Inst& I = insts[slot];
+    CInst& I = ((CInst*)insts)[slot];
     const int64_t n = (int64_t)I.nx * I.ny * I.nz;
     double s = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kDiagNT + threadIdx.x; i < n; i += (int64_t)kDiagBlocks * kDiagNT)
@@ -39,7 +39,7 @@ __global__ __launch_bounds__(kDiagNT) void field_sum_kernel(const Inst* __restri
 __global__ __launch_bounds__(kDiagNT) void vy_mode_kernel(const Inst* __restrict__ insts, int slot,
                                                           double* __restrict__ part) {
     __shared__ double sred[kDiagNT / 64 + 1];
-    const Inst& I = insts[slot];
+    CInst& I = ((CInst*)insts)[slot];
     const int64_t n = (int64_t)I.nx * I.ny * I.nz;
     double ss = 0.0, sc = 0.0, sd = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kDiagNT + threadIdx.x; i < n; i += (int64_t)kDiagBlocks * kDiagNT) {
@@ -78,7 +78,7 @@ __global__ void growth_front_kernel(const Inst* __restrict__ insts, int slot, co
                                     int ncol, double* __restrict__ out) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ncol) return;
-    const Inst& I = insts[slot];
+    CInst& I = ((CInst*)insts)[slot];
     const int ix = xs[k];
     const int z = I.nz / 2;
     const int64_t plane = (int64_t)z * I.nx * I.ny;

@@ -29,7 +29,7 @@ __device__ __forceinline__ uint64_t full_word(int w, int nt) {
     return (1ull << n) - 1ull;
 }
 
-__device__ __forceinline__ int64_t tile_cells(const Inst& I, int t) {
+__device__ __forceinline__ int64_t tile_cells(CInst& I, int t) {
     int ti = t % I.ntx, tj = (t / I.ntx) % I.nty, tk = t / (I.ntx * I.nty);
     int wx = min(I.tx, I.nx - ti * I.tx), wy = min(I.ty, I.ny - tj * I.ty), wz = min(I.tz, I.nz - tk * I.tz);
     return (int64_t)wx * wy * wz;
@@ -38,7 +38,7 @@ __device__ __forceinline__ int64_t tile_cells(const Inst& I, int t) {
 // Awake flag of tile t for the next step: OR of the detector bytes over the
 // tile and its neighbours (kind 0: faces; kind 1: 3x3 in-plane + z faces),
 // wrapping where the instance is periodic and clamping elsewhere.
-__device__ __forceinline__ int dilated3d(const Inst& I, const uint8_t* N, int t, int kind) {
+__device__ __forceinline__ int dilated3d(CInst& I, const uint8_t* N, int t, int kind) {
     const int ti = t % I.ntx, tj = (t / I.ntx) % I.nty, tk = t / (I.ntx * I.nty);
     int r = 0;
     for (int dk = -1; dk <= 1; dk++) {
@@ -123,11 +123,11 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* lds, int* total)
     return base + incl - v;
 }
 
-constexpr int kFinNT = 256;
+constexpr int kFinNT = 1024;  // one tile per thread for a full segment
 constexpr int kSegWords = kSegTiles / 64;
 
 // Per-instance end-of-step update from the summed segment partials.
-__device__ __forceinline__ void finish_instance(const Inst& J, int64_t listlen, unsigned long long c0,
+__device__ __forceinline__ void finish_instance(CInst& J, int64_t listlen, unsigned long long c0,
                                                 unsigned long long c1, unsigned long long cells, const double* cs,
                                                 double vmax, int64_t step, int64_t updates, const double* scal_in,
                                                 int rotate) {
@@ -148,13 +148,12 @@ __device__ __forceinline__ void finish_instance(const Inst& J, int64_t listlen, 
 
 // LDS of the finalize body.
 struct FinLds {
-    uint64_t s_cur[2][kSegWords];
-    uint64_t s_par[2][kSegWords];   // buffer parity words the next step sees
-    uint64_t s_new[2][kSegWords];
-    int s_pre[kSegWords + 1];
-    double sd[5];
-    int si[6];
-    unsigned long long s_cnt[3];
+    uint8_t s_pb[2][kSegTiles];     // per tile: parity the next step sees
+    uint8_t s_ab[2][kSegTiles];     // per tile: awake in the next step
+    int s_wcnt[kSegWords];          // per 64-tile word: tiles in the next worklist
+    unsigned long long s_wran[kSegWords][3];  // per word: tiles that ran (set 0, set 1), their cells
+    double sd[kFinNT / 64 + 1];
+    int si[kFinNT / 64 + 2];
     double s_cs[kMaxFields];
     double s_scal[16];
     int64_t s_step;
@@ -162,22 +161,22 @@ struct FinLds {
     int s_last;
 };
 
+// One thread per tile of the segment, one wave per 64-tile bitmap word, so the
+// whole bookkeeping needs a single workgroup barrier (plus the reductions of
+// the conserved and CFL variants).
 template <int NT>
 __device__ __forceinline__ void finalize_body(const Inst* __restrict__ insts, const Seg* __restrict__ segs,
                                               int segi, int nseg, const InstSegs* __restrict__ isegs, int n_inst,
                                               SegPart* __restrict__ parts, int32_t* __restrict__ counts,
                                               int32_t* __restrict__ offsets, uint32_t* __restrict__ done,
                                               int rotate, FinLds& F) {
-    static_assert(NT >= 64 && NT <= 256 && kSegTiles % NT == 0, "finalize layout");
-    constexpr int KT = kSegTiles / NT;  // tiles per thread
-    constexpr int WPR = NT / 64;        // words per round
-    auto& s_cur = F.s_cur;
-    auto& s_new = F.s_new;
-    auto& s_par = F.s_par;
-    auto& s_pre = F.s_pre;
+    static_assert(NT == kSegTiles, "one thread per tile of a segment");
+    auto& s_pb = F.s_pb;
+    auto& s_ab = F.s_ab;
+    auto& s_wcnt = F.s_wcnt;
+    auto& s_wran = F.s_wran;
     auto& sd = F.sd;
     auto& si = F.si;
-    auto& s_cnt = F.s_cnt;
     auto& s_cs = F.s_cs;
     auto& s_scal = F.s_scal;
     auto& s_step = F.s_step;
@@ -185,180 +184,157 @@ __device__ __forceinline__ void finalize_body(const Inst* __restrict__ insts, co
     auto& s_last = F.s_last;
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const Seg sg = segs[segi];
-    const Inst& I = insts[sg.inst];
-    const int t0 = sg.tile0, nts = sg.ntile;
+    // one segment: instance 0, all of its tiles (no dependent load of the segment table)
+    const int inst = nseg == 1 ? 0 : __builtin_amdgcn_readfirstlane(segs[segi].inst);
+    CInst& I = ((CInst*)insts)[inst];
+    const int t0 = nseg == 1 ? 0 : __builtin_amdgcn_readfirstlane(segs[segi].tile0);
+    const int nts = nseg == 1 ? I.nt : __builtin_amdgcn_readfirstlane(segs[segi].ntile);
     const int w0 = t0 >> 6, nws = (nts + 63) >> 6;
     const bool regular = I.mode == 0;
     const int nsets = I.nsets;
     const int nt = I.nt;
     const int kind = I.model == 5 ? 1 : 0;
     const bool solo = nseg == 1;
+    const int k = tid, t = t0 + k;
+    const bool valid = k < nts;
+    const bool wvalid = wid < nws;  // this wave's word exists
 
-    // ---- prologue: every independent global load of the kernel, issued together
-    uint64_t cur = 0ull, parw = 0ull;
-    const int ws = tid / kSegWords, wj = tid % kSegWords;
-    const bool wthread = tid < 2 * kSegWords && wj < nws && ws < nsets;
-    static_assert(2 * kSegWords <= 64, "word threads fit in one wave");
-    if (wthread) {
-        cur = regular ? full_word(w0 + wj, nt) : I.a_cur[ws][w0 + wj];
-        parw = I.par[ws][w0 + wj];
+    // ---- prologue: every independent global load, issued together
+    uint64_t cur0 = 0ull, cur1 = 0ull, par0 = 0ull, par1 = 0ull;
+    if (wvalid) {
+        cur0 = regular ? full_word(w0 + wid, nt) : I.a_cur[0][w0 + wid];
+        par0 = I.par[0][w0 + wid];
+        if (nsets > 1) {
+            cur1 = regular ? full_word(w0 + wid, nt) : I.a_cur[1][w0 + wid];
+            par1 = I.par[1][w0 + wid];
+        }
     }
-    int a0[KT], a1[KT];
-    {
-        const bool flat = I.ntz == 1;
-        const int ntx = I.ntx, nty = I.nty;
-#pragma unroll
-        for (int i = 0; i < KT; i++) {
-            const int k = i * NT + tid;
-            const int t = t0 + k;
-            a0[i] = 0;
-            a1[i] = 0;
-            if (k >= nts || !rotate || regular) continue;
-            if (flat) {
-                const int tj = t / ntx, ti = t - tj * ntx;
-                const Nb2 n = nb2_of(ti, tj, ntx, nty, I.bc_x, I.bc_y);
-                a0[i] = dilated2d(I.nflag[0], ti, tj, ntx, n, kind);
-                if (nsets > 1) a1[i] = dilated2d(I.nflag[1], ti, tj, ntx, n, kind);
-            } else {
-                a0[i] = dilated3d(I, I.nflag[0], t, kind);
-                if (nsets > 1) a1[i] = dilated3d(I, I.nflag[1], t, kind);
-            }
+    int a0 = 0, a1 = 0;
+    if (valid && rotate && !regular) {
+        if (I.ntz == 1) {
+            const int ntx = I.ntx;
+            const int tj = t / ntx, ti = t - tj * ntx;
+            const Nb2 n = nb2_of(ti, tj, ntx, I.nty, I.bc_x, I.bc_y);
+            a0 = dilated2d(I.nflag[0], ti, tj, ntx, n, kind);
+            if (nsets > 1) a1 = dilated2d(I.nflag[1], ti, tj, ntx, n, kind);
+        } else {
+            a0 = dilated3d(I, I.nflag[0], t, kind);
+            if (nsets > 1) a1 = dilated3d(I, I.nflag[1], t, kind);
         }
     }
     if (solo && tid < 16) s_scal[tid] = I.scal[tid];
     if (solo && tid == 16) s_step = I.ctr[0];
     if (solo && tid == 17) s_upd = I.ctr[1];
-    if (tid < 3) s_cnt[tid] = 0ull;
-    if (wthread) {
-        s_cur[ws][wj] = cur;
-        s_par[ws][wj] = rotate ? (parw ^ cur) : parw;
-    } else if (tid < 2 * kSegWords) {
-        s_cur[ws][wj] = 0ull;
-        s_par[ws][wj] = 0ull;
-    }
-    __syncthreads();
 
-    if (rotate) {
-        // (a) statistics of the executed step, parity flip
-        if (wthread) I.par[ws][w0 + wj] = parw ^ cur;
-        if (tid < nws) {
-            const uint64_t u = s_cur[0][tid] | s_cur[1][tid];
-            unsigned long long cells = 0;
-            if (I.full_tiles) {
-                cells = (unsigned long long)__popcll(u) * (unsigned long long)(I.tx * I.ty * I.tz);
-            } else {
-                uint64_t m = u;
-                while (m) {
-                    const int b = __ffsll((unsigned long long)m) - 1;
-                    cells += (unsigned long long)tile_cells(I, (w0 + tid) * 64 + b);
-                    m &= m - 1ull;
-                }
-            }
-            atomicAdd(&s_cnt[0], (unsigned long long)__popcll(s_cur[0][tid]));
-            atomicAdd(&s_cnt[1], (unsigned long long)__popcll(s_cur[1][tid]));
-            atomicAdd(&s_cnt[2], cells);
+    // ---- this tile: ran in the executed step?  awake in the next?  parity next?
+    const int ran0 = (int)((cur0 >> lane) & 1ull), ran1 = (int)((cur1 >> lane) & 1ull);
+    int b0, b1;
+    if (rotate && !regular) {
+        b0 = a0;
+        b1 = a1;
+    } else {
+        // regular: every tile every step; rebuild (rotate = 0): the current sets
+        b0 = valid && ran0;
+        b1 = valid && ran1;
+    }
+    const uint64_t pn0 = rotate ? (par0 ^ cur0) : par0;
+    const uint64_t pn1 = rotate ? (par1 ^ cur1) : par1;
+    const uint64_t nw0 = __ballot(b0);
+    const uint64_t nw1 = __ballot(b1);
+    const uint64_t uni = nw0 | nw1;
+    s_ab[0][k] = (uint8_t)b0;
+    s_ab[1][k] = (uint8_t)b1;
+    s_pb[0][k] = (uint8_t)((pn0 >> lane) & 1ull);
+    s_pb[1][k] = (uint8_t)((pn1 >> lane) & 1ull);
+    if (rotate && wvalid && lane == 0) {
+        I.par[0][w0 + wid] = pn0;
+        if (nsets > 1) I.par[1][w0 + wid] = pn1;
+        if (!regular) {
+            I.a_cur[0][w0 + wid] = nw0;
+            if (nsets > 1) I.a_cur[1][w0 + wid] = nw1;
         }
-        // conserved change sums of the tiles that ran (sleeping tiles contribute 0)
-        for (int f = 0; f < I.ncons; f++) {
-            double csum = 0.0;
+    }
+    // per-word statistics of the executed step
+    unsigned long long cells = 0ull;
+    if (rotate && valid && (ran0 | ran1)) cells = I.full_tiles ? (unsigned long long)(I.tx * I.ty * I.tz)
+                                                               : (unsigned long long)tile_cells(I, t);
 #pragma unroll
-            for (int i = 0; i < KT; i++) {
-                const int k = i * NT + tid;
-                if (k < nts && ((s_cur[0][k >> 6] >> (k & 63)) & 1ull)) csum += I.tpart[(int64_t)f * nt + t0 + k];
-            }
+    for (int off = 32; off >= 1; off >>= 1) cells += __shfl_down(cells, off, 64);
+    if (lane == 0) {
+        s_wcnt[wid] = __popcll(uni);
+        s_wran[wid][0] = (unsigned long long)__popcll(cur0);
+        s_wran[wid][1] = (unsigned long long)__popcll(cur1);
+        s_wran[wid][2] = cells;
+    }
+    // conserved change sums of the tiles that ran (sleeping tiles contribute 0)
+    if (rotate) {
+        for (int f = 0; f < I.ncons; f++) {
+            const double csum = (valid && ran0) ? I.tpart[(int64_t)f * nt + t] : 0.0;
             const double tot = block_sum<NT>(csum, sd);
             if (tid == 0) s_cs[f] = tot;
         }
     }
-
-    // (b) next awake set, one ballot per 64-tile word
-#pragma unroll
-    for (int i = 0; i < KT; i++) {
-        const int k = i * NT + tid;
-        int b0 = a0[i], b1 = a1[i];
-        if (k < nts && (regular || !rotate)) {
-            b0 = regular ? 1 : (int)((s_cur[0][k >> 6] >> (k & 63)) & 1ull);
-            b1 = regular ? (nsets > 1) : (int)((s_cur[1][k >> 6] >> (k & 63)) & 1ull);
-        }
-        const uint64_t w0b = __ballot(b0);
-        const uint64_t w1b = __ballot(b1);
-        if (lane == 0) {
-            s_new[0][i * WPR + wid] = w0b;
-            s_new[1][i * WPR + wid] = w1b;
-        }
-    }
-    __syncthreads();
-    if (rotate && !regular && wthread) I.a_cur[ws][w0 + wj] = s_new[ws][wj];
     // KH: max wave speed the next step's CFL sees (fresh for awake tiles, stale otherwise)
     double vmax = 0.0;
     if (rotate && I.model == 5 && I.dt_mode == 1) {
-        for (int k = tid; k < nts; k += NT) {
-            const int a = (int)((s_new[0][k >> 6] >> (k & 63)) & 1ull);
-            vmax = fmax(vmax, a ? I.tspeed[t0 + k] : I.tspeed[nt + t0 + k]);
-        }
+        if (valid) vmax = b0 ? I.tspeed[t] : I.tspeed[nt + t];
         vmax = block_max<NT>(vmax, sd);
     }
-
-    // (c) compaction of the union set of the segment
-    if (tid < kSegWords) {
-        // exclusive prefix of the word popcounts (16 lanes of wave 0)
-        const int c = __popcll(s_new[0][tid] | s_new[1][tid]);
-        int incl = c;
-#pragma unroll
-        for (int off = 1; off < kSegWords; off <<= 1) {
-            const int y = __shfl_up(incl, off, 64);
-            if (tid >= off) incl += y;
-        }
-        s_pre[tid] = incl - c;
-        if (tid == kSegWords - 1) s_pre[kSegWords] = incl;
-    }
     __syncthreads();
-    const bool flat2 = I.ntz == 1;
-#pragma unroll
-    for (int i = 0; i < KT; i++) {
-        const int k = i * NT + tid;
-        if (k >= nts) break;
-        const int j = k >> 6, b = k & 63;
-        const uint64_t u = s_new[0][j] | s_new[1][j];
-        if (!((u >> b) & 1ull)) continue;
-        const int pos = t0 + s_pre[j] + __popcll(u & ((1ull << b) - 1ull));
-        I.seg[pos] = t0 + k;
-        if (flat2) {
-            // neighbour record: resolved here when all nine owners are in this segment
+
+    // ---- compaction of the union set (ascending tiles) and the neighbour records
+    int base = 0, listlen = 0;
+    for (int w = 0; w < nws; w++) {
+        const int c = s_wcnt[w];
+        if (w < wid) base += c;
+        listlen += c;
+    }
+    if ((uni >> lane) & 1ull) {
+        const int pos = t0 + base + __popcll(uni & ((1ull << lane) - 1ull));
+        I.seg[pos] = t;
+        if (I.ntz == 1) {
+            // resolved here when all nine owners are in this segment
             uint64_t rec = kRecValid;
+            const int tj = t / I.ntx, ti = t - tj * I.ntx;
             for (int d = 0; d < 9; d++) {
-                const int o = nb_owner2d(I, t0 + k, I.tx, d) - t0;
+                const int o = nb_owner2d(I, ti, tj, d) - t0;
                 if (o < 0 || o >= nts) {
                     rec = 0ull;
                     break;
                 }
                 for (int s2 = 0; s2 < nsets; s2++) {
-                    rec |= ((s_par[s2][o >> 6] >> (o & 63)) & 1ull) << (s2 * 18 + d);
-                    rec |= ((s_new[s2][o >> 6] >> (o & 63)) & 1ull) << (s2 * 18 + 9 + d);
+                    rec |= (uint64_t)s_pb[s2][o] << (s2 * 18 + d);
+                    rec |= (uint64_t)s_ab[s2][o] << (s2 * 18 + 9 + d);
                 }
             }
             I.nbrec[pos] = rec;
         }
     }
+    unsigned long long c0 = 0, c1 = 0, ccells = 0;
+    if (tid == 0) {
+        for (int w = 0; w < nws; w++) {
+            c0 += s_wran[w][0];
+            c1 += s_wran[w][1];
+            ccells += s_wran[w][2];
+        }
+    }
 
-    // (d) per-instance reductions and merged offsets
+    // ---- per-instance reductions and merged offsets
     if (solo) {
         if (tid == 0) {
             double cs[kMaxFields] = {s_cs[0], s_cs[1], s_cs[2], s_cs[3]};
-            finish_instance(I, s_pre[kSegWords], s_cnt[0], s_cnt[1], s_cnt[2], cs, vmax, s_step, s_upd, s_scal,
-                            rotate);
+            finish_instance(I, listlen, c0, c1, ccells, cs, vmax, s_step, s_upd, s_scal, rotate);
             offsets[0] = 0;
-            offsets[1] = s_pre[kSegWords];
+            offsets[1] = listlen;
         }
         return;
     }
     if (tid == 0) {
-        counts[segi] = s_pre[kSegWords];
+        counts[segi] = listlen;
         if (rotate) {
-            parts[segi].c0 = s_cnt[0];
-            parts[segi].c1 = s_cnt[1];
-            parts[segi].cells = s_cnt[2];
+            parts[segi].c0 = c0;
+            parts[segi].c1 = c1;
+            parts[segi].cells = ccells;
             for (int f = 0; f < kMaxFields; f++) parts[segi].csum[f] = f < I.ncons ? s_cs[f] : 0.0;
             parts[segi].vmax = vmax;
         }
@@ -370,29 +346,29 @@ __device__ __forceinline__ void finalize_body(const Inst* __restrict__ insts, co
     if (!s_last) return;
     __threadfence();
     for (int i = tid; i < n_inst; i += NT) {
-        const Inst& J = insts[i];
+        CInst& J = ((CInst*)insts)[i];
         const InstSegs r = isegs[i];
-        int64_t listlen = 0;
-        unsigned long long c0 = 0, c1 = 0, cells = 0;
+        int64_t ll = 0;
+        unsigned long long q0 = 0, q1 = 0, qc = 0;
         double cs[kMaxFields] = {0.0, 0.0, 0.0, 0.0};
         double vm = 0.0;
         for (int q = r.first; q < r.first + r.count; q++) {
-            listlen += __hip_atomic_load(&counts[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ll += __hip_atomic_load(&counts[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (rotate) {
-                c0 += parts[q].c0;
-                c1 += parts[q].c1;
-                cells += parts[q].cells;
+                q0 += parts[q].c0;
+                q1 += parts[q].c1;
+                qc += parts[q].cells;
                 for (int f = 0; f < J.ncons; f++) cs[f] += parts[q].csum[f];
                 vm = fmax(vm, parts[q].vmax);
             }
         }
         double sc[16];
-        for (int k = 0; k < 16; k++) sc[k] = J.scal[k];
-        finish_instance(J, listlen, c0, c1, cells, cs, vm, J.ctr[0], J.ctr[1], sc, rotate);
+        for (int kk = 0; kk < 16; kk++) sc[kk] = J.scal[kk];
+        finish_instance(J, ll, q0, q1, qc, cs, vm, J.ctr[0], J.ctr[1], sc, rotate);
     }
     int running = 0;
-    for (int base = 0; base < nseg; base += NT) {
-        const int i = base + tid;
+    for (int b = 0; b < nseg; b += NT) {
+        const int i = b + tid;
         const int v = i < nseg ? __hip_atomic_load(&counts[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
         int tot;
         const int ex = block_exclusive_scan<NT>(v, si, &tot);
@@ -420,7 +396,7 @@ __global__ __launch_bounds__(kFinNT) void finalize_kernel(const Inst* __restrict
 // Current value of one field: the buffer the tile parity selects, minus the
 // conserved shift accumulated since the tile last ran.
 __global__ void materialize_kernel(const Inst* __restrict__ insts, int slot, int field, double* __restrict__ dst) {
-    const Inst& I = insts[slot];
+    CInst& I = ((CInst*)insts)[slot];
     const int64_t n = (int64_t)I.nx * I.ny * I.nz;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         dst[i] = current_value(I, field, i);
@@ -429,7 +405,7 @@ __global__ void materialize_kernel(const Inst* __restrict__ insts, int slot, int
 // After an upload: both buffers hold the new values, so the shift epochs of
 // every tile restart at the current cumulative shift.
 __global__ void reset_epochs_kernel(const Inst* __restrict__ insts, int slot) {
-    const Inst& I = insts[slot];
+    CInst& I = ((CInst*)insts)[slot];
     for (int f = 0; f < I.ncons; f++) {
         const double S = I.scal[8 + f];
         for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * I.nt; i += gridDim.x * blockDim.x)
@@ -439,7 +415,7 @@ __global__ void reset_epochs_kernel(const Inst* __restrict__ insts, int slot) {
 
 // Non-finite check over the current values of all fields.
 __global__ void finite_kernel(const Inst* __restrict__ insts, int slot, int* __restrict__ bad) {
-    const Inst& I = insts[slot];
+    CInst& I = ((CInst*)insts)[slot];
     const int64_t n = (int64_t)I.nx * I.ny * I.nz;
     for (int f = 0; f < I.nfields; f++) {
         const int set = I.field_set[f];

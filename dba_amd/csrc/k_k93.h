@@ -18,7 +18,7 @@
 
 namespace dba {
 
-__device__ __forceinline__ void k93_aniso(const Inst& I, double gx, double gy, double* eps, double* a) {
+__device__ __forceinline__ void k93_aniso(CInst& I, double gx, double gy, double* eps, double* a) {
     double g2 = gx * gx + gy * gy;
     double cj = 1.0, sj = 0.0;
     if (g2 != 0.0) {
@@ -57,24 +57,24 @@ __global__ __launch_bounds__(NT) void k93_step_kernel(const WorkList wl) {
     const int G = gridDim.x;
     const int tid = threadIdx.x;
     for (int item = xcd_remap(blockIdx.x, G);; item += G) {
-        const Inst* Ip;
+        CInst* Ip;
         int t;
         uint64_t rec;
         if (!next_item(wl, item, total, &Ip, &t, &rec)) break;
-        const Inst& I = *Ip;
+        CInst& I = *Ip;
         const int nx = I.nx, ny = I.ny, ntx = I.ntx, bcx = I.bc_x, bcy = I.bc_y;
         const int x0 = (t % ntx) * TB, y0 = (t / ntx) * TB;
         const bool regular = I.mode == 0;
-        const double* __restrict__ p0 = I.buf[0][0];
-        const double* __restrict__ p1 = I.buf[1][0];
-        const double* __restrict__ T0 = I.buf[0][1];
-        const double* __restrict__ T1 = I.buf[1][1];
+        const gdouble* __restrict__ p0 = I.buf[0][0];
+        const gdouble* __restrict__ p1 = I.buf[1][0];
+        const gdouble* __restrict__ T0 = I.buf[0][1];
+        const gdouble* __restrict__ T1 = I.buf[1][1];
         nb_fill<TB>(nb, I, t, x0, y0, regular, 2, 0, rec);
         __syncthreads();
         const int ap = nb.act[0][4], at = nb.act[1][4];
         const int pp0 = nb.par[0][4], pp1 = nb.par[1][4];
-        double* __restrict__ pnew = pp0 ? I.buf[0][0] : I.buf[1][0];
-        double* __restrict__ Tnew = pp1 ? I.buf[0][1] : I.buf[1][1];
+        gdouble* __restrict__ pnew = pp0 ? I.buf[0][0] : I.buf[1][0];
+        gdouble* __restrict__ Tnew = pp1 ? I.buf[0][1] : I.buf[1][1];
 
         // stage phi (2-cell halo) when p-awake, T (1-cell halo) when T-awake,
         // and phi / T of the tile itself always (the T update needs phi' - phi)

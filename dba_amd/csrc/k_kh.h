@@ -88,11 +88,11 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
     const int G = gridDim.x;
     const int tid = threadIdx.x;
     for (int item = xcd_remap(blockIdx.x, G);; item += G) {
-        const Inst* Ip;
+        CInst* Ip;
         int t;
         uint64_t rec;
         if (!next_item(wl, item, total, &Ip, &t, &rec)) break;
-        const Inst& I = *Ip;
+        CInst& I = *Ip;
         const int nx = I.nx, ny = I.ny, ntx = I.ntx;
         const int x0 = (t % ntx) * TB, y0 = (t / ntx) * TB;
         const bool regular = I.mode == 0;
@@ -359,7 +359,7 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
 // a CFL run has its dt; the finalize then keeps them current.
 template <int TB>
 __global__ void kh_prime_kernel(const Inst* __restrict__ insts, int slot) {
-    const Inst& I = insts[slot];
+    CInst& I = ((CInst*)insts)[slot];
     KhC c;
     c.g = I.gas_gamma;
     c.dx = I.dx;
@@ -391,7 +391,7 @@ __global__ void kh_prime_kernel(const Inst* __restrict__ insts, int slot) {
 }
 
 __global__ void kh_dt_kernel(const Inst* __restrict__ insts, int slot) {
-    const Inst& I = insts[slot];
+    CInst& I = ((CInst*)insts)[slot];
     __shared__ double sred[5];
     double vm = 0.0;
     for (int t = threadIdx.x; t < I.nt; t += blockDim.x) vm = fmax(vm, I.tspeed[t]);

@@ -23,7 +23,7 @@ struct NbTable3 {
 };
 
 template <int TB, int TZ>
-__device__ __forceinline__ void nb_fill3(NbTable3& nb, const Inst& I, int x0, int y0, int z0, bool regular,
+__device__ __forceinline__ void nb_fill3(NbTable3& nb, CInst& I, int x0, int y0, int z0, bool regular,
                                          int nsets, int ncons) {
     const int d = threadIdx.x;
     if (d >= 27) return;
@@ -90,11 +90,11 @@ __global__ __launch_bounds__(64) void modelb_step_kernel(const WorkList wl) {
     const int G = gridDim.x;
     const int tid = threadIdx.x;
     for (int item = xcd_remap(blockIdx.x, G);; item += G) {
-        const Inst* Ip;
+        CInst* Ip;
         int t;
         uint64_t rec;
         if (!next_item(wl, item, total, &Ip, &t, &rec, false)) break;
-        const Inst& I = *Ip;
+        CInst& I = *Ip;
         const int nx = I.nx, ny = I.ny, nz = I.nz;
         const int x0 = (t % I.ntx) * TB, y0 = ((t / I.ntx) % I.nty) * TB, z0 = (t / (I.ntx * I.nty)) * TZ;
         const bool regular = I.mode == 0;
@@ -210,11 +210,11 @@ __global__ __launch_bounds__(64) void k94_step_kernel(const WorkList wl) {
     const int G = gridDim.x;
     const int tid = threadIdx.x;
     for (int item = xcd_remap(blockIdx.x, G);; item += G) {
-        const Inst* Ip;
+        CInst* Ip;
         int t;
         uint64_t rec;
         if (!next_item(wl, item, total, &Ip, &t, &rec, false)) break;
-        const Inst& I = *Ip;
+        CInst& I = *Ip;
         const int nx = I.nx, ny = I.ny, nz = I.nz;
         const int x0 = (t % I.ntx) * TB, y0 = ((t / I.ntx) % I.nty) * TB, z0 = (t / (I.ntx * I.nty)) * TZ;
         const bool regular = I.mode == 0;
