@@ -236,7 +236,7 @@ def bench_serve(args):
     with Context(0) as ctx:
         free, total = ctx.device_memory()
         be = serve.LibraryBackend(ctx)
-        srv = serve.Server(be, budget, chunk=args.chunk, tiles=tiles, target_occupancy=0.8, gain=2.0)
+        srv = serve.Server(be, budget, chunk=args.chunk, tiles=tiles, target_occupancy=0.8, gain=2.0, log=log)
         done = srv.run(rs, rate=args.rate)
         rep = serve.report(done, srv.wall, srv.occupancy)
         log(f"served {rep['completed']} in {srv.wall:.1f} s, p50 {rep['p50_latency_s']:.3f} s, "

@@ -200,8 +200,11 @@ dba_status validate(const dba_params& p) {
         return fail(DBA_ERR_INVALID, "aniso_j must be in [1,16]");
     if (p.model == DBA_MODEL_DENDRITE_K93 && p.mode == DBA_MODE_CONSERVED)
         return fail(DBA_ERR_INVALID, "conserved DBA is defined for Model B and KH only");
-    if (p.threads != 0 && p.threads != 64 && p.threads != 128 && p.threads != 256)
-        return fail(DBA_ERR_INVALID, "threads must be 0, 64, 128 or 256");
+    const bool wide = p.threads == 512 || p.threads == 1024;
+    if (p.threads != 0 && p.threads != 64 && p.threads != 128 && p.threads != 256 && !wide)
+        return fail(DBA_ERR_INVALID, "threads must be 0, 64, 128, 256, 512 or 1024");
+    if (wide && p.model != DBA_MODEL_CH && p.model != DBA_MODEL_DENDRITE_K93)
+        return fail(DBA_ERR_INVALID, "512 and 1024 threads are for the CH and Kobayashi-93 models");
     if (p.dx <= 0.0 || p.dt <= 0.0) return fail(DBA_ERR_INVALID, "dx and dt must be positive");
     return DBA_OK;
 }
@@ -344,10 +347,12 @@ void evict_graphs(dba_ctx* ctx) {
     }
 }
 
+// Workgroup size per (model, tile) measured best on MI355X (CH and Kobayashi-93
+// grids of 1024^2 to 4096^2); the workgroup size never changes results.
 int default_threads(int model, int tile) {
-    (void)model;
     if (tile <= 8) return 64;
-    return 256;
+    if (tile == 16) return (model == DBA_MODEL_CH || model == DBA_MODEL_DENDRITE_K93) ? 128 : 256;
+    return model == DBA_MODEL_DENDRITE_K93 ? 512 : 256;
 }
 
 // KH: wave speeds of every tile from the current state, and the first CFL dt.
@@ -445,7 +450,7 @@ void launch_step_tb(const Group* g, hipStream_t s) {
             hipLaunchKernelGGL((k93_step_kernel<TB, NT>), grid, block, 0, s, wl_of(g));
             break;
         case DBA_MODEL_KH:
-            if constexpr (TB <= 16) {
+            if constexpr (TB <= 16 && NT <= 256) {
                 hipLaunchKernelGGL((kh_step_kernel<TB, NT>), grid, block, 0, s, wl_of(g));
             }
             break;
@@ -459,6 +464,8 @@ void launch_step_nt(const Group* g, hipStream_t s) {
     switch (g->threads) {
         case 64: launch_step_tb<TB, 64>(g, s); break;
         case 128: launch_step_tb<TB, 128>(g, s); break;
+        case 512: launch_step_tb<TB, 512>(g, s); break;
+        case 1024: launch_step_tb<TB, 1024>(g, s); break;
         default: launch_step_tb<TB, 256>(g, s); break;
     }
 }

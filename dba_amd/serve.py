@@ -216,10 +216,15 @@ class Server:
                 new[r.rid] = [h, r.steps, r, clock() - t0, fp]
             return new
 
+        last_log = 0.0
         while todo or queue or resident:
             now = clock() - t0
             if max_wall is not None and now > max_wall:
                 break
+            if self.log and now - last_log >= 10.0:
+                last_log = now
+                self.log(f"serve t={now:.0f}s done={len(done)} resident={len(resident)} queued={len(queue)} "
+                         f"used={self.used / self.capacity:.2f}" + (f" rate={rate:.2f}/s" if rate else ""))
             arrivals(now)
             k = 0
             if resident:

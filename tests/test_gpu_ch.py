@@ -36,7 +36,7 @@ def run_pair(ctx, p, c0, steps):
 
 @pytest.mark.parametrize("mode", [abi.MODE_REGULAR, abi.MODE_DBA, abi.MODE_CONSERVED])
 @pytest.mark.parametrize("stencil", [5, 9])
-@pytest.mark.parametrize("tile,threads", [(8, 64), (16, 128), (32, 256), (16, 256)])
+@pytest.mark.parametrize("tile,threads", [(8, 64), (16, 128), (32, 256), (16, 256), (32, 512), (32, 1024), (16, 0)])
 def test_ch_matches_oracle(ctx, mode, stencil, tile, threads):
     n = max(128, 8 * tile)
     p = configs.ch_params(n=n, tile=tile, mode=mode, stencil=stencil, eps=1e-3, threads=threads,

@@ -22,7 +22,7 @@ TOL = 1e-10
 
 
 @pytest.mark.parametrize("mode", [abi.MODE_REGULAR, abi.MODE_DBA])
-@pytest.mark.parametrize("tile,threads", [(16, 256), (8, 64), (32, 256), (16, 128)])
+@pytest.mark.parametrize("tile,threads", [(16, 256), (8, 64), (32, 256), (16, 128), (32, 512), (16, 1024), (32, 0)])
 def test_k93_matches_oracle(ctx, mode, tile, threads):
     n = 128
     p = configs.dendrite_k93_params(n=n, tile=tile, mode=mode, threads=threads)
