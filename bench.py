@@ -181,10 +181,13 @@ def bench_batch(args, K):
             i.destroy()
         st16, _ = batch.run_batched(ctx, items, K, tile=16, threads=0, chunk=args.chunk, fields_cache=cache)
         log(f"batched 16x16: {st16.wall_s:.3f} s")
-        seq, _ = batch.run_sequential(ctx, items, K, tile=16, threads=0, chunk=args.chunk, fields_cache=cache)
-        log(f"sequential: {seq.wall_s:.3f} s, {seq.active_rate:.4g} active/s")
-        plain, _ = batch.run_sequential(ctx, items, K, tile=16, threads=0, fields_cache=cache, plain=True)
-        log(f"sequential plain: {plain.wall_s:.3f} s, {plain.active_rate:.4g} active/s")
+        if args.no_seq:
+            seq = plain = None
+        else:
+            seq, _ = batch.run_sequential(ctx, items, K, tile=16, threads=0, chunk=args.chunk, fields_cache=cache)
+            log(f"sequential: {seq.wall_s:.3f} s, {seq.active_rate:.4g} active/s")
+            plain, _ = batch.run_sequential(ctx, items, K, tile=16, threads=0, fields_cache=cache, plain=True)
+            log(f"sequential plain: {plain.wall_s:.3f} s, {plain.active_rate:.4g} active/s")
     out = {
         "metric": "active cell-updates/s of the batched 64-instance dendrite sweep (+ vs plain sequential DBA)",
         "value": st.active_rate, "unit": "cell-updates/s", "n_gpus": 1, "steps": K, "warmup": 0,
@@ -194,15 +197,17 @@ def bench_batch(args, K):
                    "tile": tile, "threads": nt, "graph_chunk": args.chunk, **batch.summary(items)},
         "full_grid_updates_per_s": st.full_rate,
         "active_fraction": st.active_updates / st.full_updates,
-        "batched_vs_sequential_plain": st.active_rate / plain.active_rate,
-        "batched_vs_sequential_graph": st.active_rate / seq.active_rate,
-        "sequential_plain": {"value": plain.active_rate, "wall_s": plain.wall_s, "tile": 16, "threads": "default",
-                             "note": "one instance at a time, one step launched and waited for per time step "
-                                     "(the reference loop's shape), stepping time only"},
-        "sequential_graph": {"value": seq.active_rate, "wall_s": seq.wall_s, "tile": 16, "threads": "default",
-                             "note": "one instance at a time, graph-replayed chunks of steps, stepping time only"},
+        "batched_vs_sequential_plain": st.active_rate / plain.active_rate if plain else None,
+        "batched_vs_sequential_graph": st.active_rate / seq.active_rate if seq else None,
+        "sequential_plain": None if plain is None else {
+            "value": plain.active_rate, "wall_s": plain.wall_s, "tile": 16, "threads": "default",
+            "note": "one instance at a time, one step launched and waited for per time step "
+                    "(the reference loop's shape), stepping time only"},
+        "sequential_graph": None if seq is None else {
+            "value": seq.active_rate, "wall_s": seq.wall_s, "tile": 16, "threads": "default",
+            "note": "one instance at a time, graph-replayed chunks of steps, stepping time only"},
         "batched_16x16": {"value": st16.active_rate, "wall_s": st16.wall_s,
-                          "vs_sequential_wall": seq.wall_s / st16.wall_s},
+                          "vs_sequential_wall": seq.wall_s / st16.wall_s if seq else None},
         "autotune": [{"tile": a, "threads": b, "ms_per_step": c, "active_rate": d} for a, b, c, d in table],
         "ic_host_s": ic_s, "finite": bool(finite), "roofline": roof,
     }
@@ -308,6 +313,7 @@ def main():
     ap.add_argument("--requests", type=int, default=300, help="serve: requests in the stream")
     ap.add_argument("--rate", type=float, default=4.0, help="serve: initial arrival rate (1/s)")
     ap.add_argument("--budget-gib", type=float, default=4.0, help="serve: device memory budget for admission")
+    ap.add_argument("--no-seq", action="store_true", help="batch: skip the one-instance-at-a-time loops")
     ap.add_argument("--seq-budget-s", type=float, default=30.0, help="serve: time for the sequential sample")
     args = ap.parse_args()
     if args.gpus != 1:
