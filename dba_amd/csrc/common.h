@@ -317,4 +317,59 @@ __device__ __forceinline__ bool next_item(const WorkList& w, int item, int total
     return true;
 }
 
+// Grid-stride walk over the worklist with an XCD-aware deal: workgroups are
+// issued to the 8 XCDs in turn (b % 8), so each round of G items is split into
+// 8 contiguous ranges, range x going to the workgroups of XCD x. A round with
+// fewer than G items (a sparse list) is still spread over all 8 XCDs. For a
+// single-instance list, the first item of a full round is read before the list
+// length arrives (it is the same item either way), so both loads overlap.
+struct WorkCursor {
+    int base;       // first item of the next round
+    int spec_item;  // item whose entry was read ahead (-1: none)
+    int spec_t;
+    uint64_t spec_rec;
+};
+
+__device__ __forceinline__ int xcd_deal(int b, int G, int base, int cnt) {
+    if ((G & 7) != 0) return b < cnt ? base + b : -1;
+    const int per = (cnt + 7) >> 3, slot = b >> 3, x = b & 7;
+    if (slot >= per) return -1;
+    const int item = base + x * per + slot;
+    return item < base + cnt ? item : -1;
+}
+
+__device__ __forceinline__ void work_begin(const WorkList& w, WorkCursor& c, bool want_rec) {
+    c.base = 0;
+    c.spec_item = -1;
+    if (w.solo_nt > 0) {
+        const int G = gridDim.x;
+        const int item = xcd_deal(blockIdx.x, G, 0, G);  // the item of a full first round
+        if (item >= 0 && item < w.solo_nt) {
+            c.spec_item = item;
+            c.spec_t = w.solo_seg[item];
+            c.spec_rec = want_rec ? w.solo_rec[item] : 0ull;
+        }
+    }
+}
+
+__device__ __forceinline__ bool work_next(const WorkList& w, WorkCursor& c, int total, CInst** I, int* t,
+                                          uint64_t* rec, bool want_rec = true) {
+    const int G = gridDim.x;
+    while (c.base < total) {
+        const int item = xcd_deal(blockIdx.x, G, c.base, min(G, total - c.base));
+        c.base += G;
+        if (item < 0) continue;
+        if (item == c.spec_item) {
+            *I = (CInst*)w.insts;
+            *t = __builtin_amdgcn_readfirstlane(c.spec_t);
+            *rec = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)c.spec_rec)) |
+                   ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(c.spec_rec >> 32)) << 32);
+            c.spec_item = -1;
+            return true;
+        }
+        return next_item(w, item, total, I, t, rec, want_rec);
+    }
+    return false;
+}
+
 }  // namespace dba

@@ -169,13 +169,14 @@ __global__ __launch_bounds__(NT) void ch_step_kernel(const WorkList wl) {
     __shared__ int sflag[NT / 64 + 1];
     __shared__ NbTable nb;
 
+    WorkCursor cur;
+    work_begin(wl, cur, true);
     const int total = wl.offsets[wl.nseg];
-    const int G = gridDim.x;
-    for (int item = xcd_remap(blockIdx.x, G);; item += G) {
+    for (;;) {
         CInst* Ip;
         int t;
         uint64_t rec;
-        if (!next_item(wl, item, total, &Ip, &t, &rec)) break;
+        if (!work_next(wl, cur, total, &Ip, &t, &rec)) break;
         CInst& I = *Ip;
         if (I.stencil != 5) ch_tile<TB, NT, true>(I, t, rec, nb, sc, smu, sred, sflag);
         else ch_tile<TB, NT, false>(I, t, rec, nb, sc, smu, sred, sflag);

@@ -53,14 +53,15 @@ __global__ __launch_bounds__(NT) void k93_step_kernel(const WorkList wl) {
     __shared__ int sflag[NT / 64 + 1];
     __shared__ NbTable nb;
 
+    WorkCursor cur;
+    work_begin(wl, cur, true);
     const int total = wl.offsets[wl.nseg];
-    const int G = gridDim.x;
     const int tid = threadIdx.x;
-    for (int item = xcd_remap(blockIdx.x, G);; item += G) {
+    for (;;) {
         CInst* Ip;
         int t;
         uint64_t rec;
-        if (!next_item(wl, item, total, &Ip, &t, &rec)) break;
+        if (!work_next(wl, cur, total, &Ip, &t, &rec)) break;
         CInst& I = *Ip;
         const int nx = I.nx, ny = I.ny, ntx = I.ntx, bcx = I.bc_x, bcy = I.bc_y;
         const int x0 = (t % ntx) * TB, y0 = (t / ntx) * TB;

@@ -86,14 +86,15 @@ __global__ __launch_bounds__(64) void modelb_step_kernel(const WorkList wl) {
     __shared__ int sflag[2];
     __shared__ NbTable3 nb;
 
+    WorkCursor cur;
+    work_begin(wl, cur, false);
     const int total = wl.offsets[wl.nseg];
-    const int G = gridDim.x;
     const int tid = threadIdx.x;
-    for (int item = xcd_remap(blockIdx.x, G);; item += G) {
+    for (;;) {
         CInst* Ip;
         int t;
         uint64_t rec;
-        if (!next_item(wl, item, total, &Ip, &t, &rec, false)) break;
+        if (!work_next(wl, cur, total, &Ip, &t, &rec, false)) break;
         CInst& I = *Ip;
         const int nx = I.nx, ny = I.ny, nz = I.nz;
         const int x0 = (t % I.ntx) * TB, y0 = ((t / I.ntx) % I.nty) * TB, z0 = (t / (I.ntx * I.nty)) * TZ;
@@ -206,14 +207,15 @@ __global__ __launch_bounds__(64) void k94_step_kernel(const WorkList wl) {
     __shared__ int sflag[2];
     __shared__ NbTable3 nb;
 
+    WorkCursor cur;
+    work_begin(wl, cur, false);
     const int total = wl.offsets[wl.nseg];
-    const int G = gridDim.x;
     const int tid = threadIdx.x;
-    for (int item = xcd_remap(blockIdx.x, G);; item += G) {
+    for (;;) {
         CInst* Ip;
         int t;
         uint64_t rec;
-        if (!next_item(wl, item, total, &Ip, &t, &rec, false)) break;
+        if (!work_next(wl, cur, total, &Ip, &t, &rec, false)) break;
         CInst& I = *Ip;
         const int nx = I.nx, ny = I.ny, nz = I.nz;
         const int x0 = (t % I.ntx) * TB, y0 = ((t / I.ntx) % I.nty) * TB, z0 = (t / (I.ntx * I.nty)) * TZ;
