@@ -48,6 +48,8 @@ WORKLOADS = {
     "ch_cons": dict(model="ch", n=1024, tile=32, mode="conserved", b_alg=16, warm=500, steps=5000),
     "dendrite": dict(model="k93", n=512, tile=16, mode="dba", b_alg=32, warm=200, steps=4000),
     "kh": dict(model="kh", n=2048, tile=16, mode="dba", b_alg=64, warm=100, steps=1000),
+    "modelb": dict(model="modelb", n=256, tile=4, mode="dba", b_alg=48, warm=20, steps=200),
+    "dendrite3d": dict(model="k94", n=256, tile=4, mode="dba", b_alg=32, warm=100, steps=2000),
     "batch": dict(model="k93", b_alg=32, warm=0, steps=5000),
     "serve": dict(model="mix", b_alg=None, warm=0, steps=0),
     "copycal": dict(model="ch", b_alg=None, warm=0, steps=0),
@@ -55,6 +57,7 @@ WORKLOADS = {
 
 
 def make(workload, n=None, tile=None, threads=0):
+    import numpy as np
     from dba_amd import abi, configs
     w = dict(WORKLOADS[workload])
     if n:
@@ -68,28 +71,105 @@ def make(workload, n=None, tile=None, threads=0):
     elif w["model"] == "kh":
         p = configs.kh_params(n=w["n"], tile=w["tile"], mode=mode, threads=threads)
         fields = configs.kh_ic(w["n"])
+    elif w["model"] == "modelb":
+        n = w["n"]
+        p = configs.modelb_params(n=n, mode=mode)
+        fields = list(configs.modelb_ic(n, n, n)) + [np.zeros((n, n, n))]
+    elif w["model"] == "k94":
+        n = w["n"]
+        p = configs.dendrite_k94_params(n=n, mode=mode)
+        fields = list(configs.dendrite_k94_ic(n, n, n))
     else:
         p = configs.dendrite_k93_params(n=w["n"], tile=w["tile"], mode=mode, threads=threads)
         fields = list(configs.dendrite_k93_ic(w["n"], w["n"]))
     return p, fields
 
 
+def reference_cpu_baseline(workload, budget_s=15.0):
+    """The reference's own CPU_version step loop (compiled by oracle/Makefile.ref into
+    oracle/_ref), single-threaded as the reference runs it, on the workload's grid;
+    None when that build is absent."""
+    import ctypes as C
+    import numpy as np
+    from dba_amd import configs
+    ref = os.path.join(ROOT, "oracle", "_ref")
+    w = WORKLOADS[workload]
+    n = w["n"]
+    D = C.POINTER(C.c_double)
+    if w["model"] == "modelb":
+        path = os.path.join(ref, "libref_modelb.so")
+        if not os.path.exists(path):
+            return None
+        lib = C.CDLL(path)
+        lib.ref_modelb_run.argtypes = [C.c_int, C.c_int, C.c_int, D, C.c_int, D, D, D, D]
+        p = configs.modelb_params(n=n)
+        prm = np.array([p.dx, p.dt, p.chi_xy, p.chi_xz, p.chi_yz, p.chi_xb, p.chi_yb, p.chi_zb, p.v_x, p.v_y, p.v_z,
+                        p.r_x, p.r_y, p.r_z, p.n_exp, p.m_exp, p.epsx_sq, p.epsy_sq, p.epsz_sq, p.epsphi_sq,
+                        p.mob_x, p.mob_y, p.mob_z, p.mob_phi, p.k_eq, p.z_crit, p.p_gel, p.k_0])
+        x, y, z = (np.ascontiguousarray(a) for a in configs.modelb_ic(n, n, n))
+        phi = np.zeros_like(x)
+        ptr = lambda a: a.ctypes.data_as(D)
+
+        def run(k):
+            lib.ref_modelb_run(n, n, n, ptr(prm), k, ptr(x), ptr(y), ptr(z), ptr(phi))
+        label = "example_Reactive_ModelB/CPU_version/base_CPU.cpp"
+    elif w["model"] == "k94":
+        path = os.path.join(ref, "libref_dendrite.so")
+        if not os.path.exists(path):
+            return None
+        lib = C.CDLL(path)
+        F = C.POINTER(C.c_float)
+        lib.ref_dendrite_init.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, F, F]
+        lib.ref_dendrite_run.argtypes = [C.c_int, C.c_int, C.c_int, F, C.c_int, F, F]
+        q = configs.dendrite_k94_params(n=n)
+        pf = np.zeros((n + 2) ** 3, dtype=np.float32)
+        Tf = np.zeros_like(pf)
+        lib.ref_dendrite_init(n, n, n, 0.25, pf.ctypes.data_as(F), Tf.ctypes.data_as(F))
+        prm = np.array([q.dx, q.dt, q.delta, q.tau, q.eps_bar, q.alpha, q.gamma], dtype=np.float32)
+
+        def run(k):
+            lib.ref_dendrite_run(n, n, n, prm.ctypes.data_as(F), k, pf.ctypes.data_as(F), Tf.ctypes.data_as(F))
+        label = "example_Dendritic_growth/CPU_version/Base.cpp (float32)"
+    else:
+        return None
+    t0 = time.perf_counter()
+    run(1)
+    t1 = time.perf_counter() - t0
+    steps = max(1, min(200, int(budget_s / max(t1, 1e-6))))
+    t0 = time.perf_counter()
+    run(steps)
+    t = time.perf_counter() - t0
+    return {"value": n ** 3 * steps / t, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
+            "sample": f"{steps} regular-mode steps of the {n}^3 grid through the compiled reference CPU program "
+                      f"({label}, g++ -O2), 1 thread ({cpu_model()})"}
+
+
 def cpu_baseline(workload, budget_s=12.0, n=None):
-    """Oracle regular-mode loop on the host cores, bounded sample."""
+    """Oracle regular-mode loop on the host cores, bounded sample (the reference's
+    own CPU program for the 3D reference workloads when it is built)."""
     from dba_amd import abi
     from oracle.oracle import OracleSim
+    if n is None:
+        r = reference_cpu_baseline(workload)
+        if r is not None:
+            return r
     p, fields = make(workload, n=n)
     p.mode = abi.MODE_REGULAR
-    ncell = p.nx * p.ny
+    ncell = p.nx * p.ny * p.nz
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     cores = max(1, min(cores, 16))
+    if p.nz > 1:
+        cores = 1  # the oracle steps the 3D models serially
     res = {}
     for label, nth in (("serial", 1), ("threads", cores)):
+        if label == "threads" and nth == 1:
+            res[label] = res["serial"]
+            continue
         o = OracleSim(p)
         for f, a in enumerate(fields):
             o.set_field(f, a)
-        t = o.bench(2, nth)  # calibrate
-        steps = max(2, int(budget_s / 2 / max(t / 2, 1e-6)))
+        t = o.bench(1 if p.nz > 1 else 2, nth)  # calibrate
+        steps = max(1, int(budget_s / 2 / max(t / (1 if p.nz > 1 else 2), 1e-6)))
         steps = min(steps, 2000)
         t = o.bench(steps, nth)
         res[label] = (ncell * steps / t, steps, nth)
@@ -97,7 +177,7 @@ def cpu_baseline(workload, budget_s=12.0, n=None):
     v, steps, nth = res["threads"]
     return {
         "value": v, "unit": "cell-updates/s", "cores": nth, "kind": "port",
-        "sample": f"{steps} regular-mode steps of the {p.nx}x{p.ny} grid, {nth} y-slab threads ({cpu_model()})",
+        "sample": f"{steps} regular-mode steps of the {p.nx}x{p.ny}x{p.nz} grid, {nth} y-slab threads ({cpu_model()})",
         "serial_value": res["serial"][0],
         "serial_sample": f"{res['serial'][1]} regular-mode steps, 1 thread",
     }
@@ -335,7 +415,7 @@ def main():
         print(json.dumps(copy_calibration()))
         return
     p, fields = make(args.workload, n=args.n, tile=args.tile, threads=args.threads)
-    ncell = p.nx * p.ny
+    ncell = p.nx * p.ny * p.nz
 
     with Context(0) as ctx:
         inst = ctx.create(p)
@@ -388,8 +468,9 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"{args.workload}: {p.nx}x{p.ny} FP64, {p.tile_x}x{p.tile_y} tiles, {w['mode']}",
-            "grid": [p.nx, p.ny], "tile": p.tile_x, "threads": p.threads or "default", "mode": w["mode"],
+            "workload": f"{args.workload}: {p.nx}x{p.ny}" + (f"x{p.nz}" if p.nz > 1 else "") +
+                        f" FP64, {p.tile_x}x{p.tile_y}" + (f"x{p.tile_z}" if p.nz > 1 else "") + f" tiles, {w['mode']}",
+            "grid": [p.nx, p.ny, p.nz], "tile": p.tile_x, "threads": p.threads or "default", "mode": w["mode"],
             "graph_chunk": chunk,
         },
         "full_grid_updates_per_s": K * ncell / wall_s,

@@ -996,6 +996,14 @@ double ora_bench(ora_sim* s, int nsteps, int nthreads) {
         s->steps += nsteps;
         return (double)(b.tv_sec - a.tv_sec) + 1e-9 * (double)(b.tv_nsec - a.tv_nsec);
     }
+    if (s->p.model == DBA_MODEL_DENDRITE_K94 || s->p.model == DBA_MODEL_MODELB) {
+        /* 3D reference models: serial step loop */
+        struct timespec a, b;
+        clock_gettime(CLOCK_MONOTONIC, &a);
+        ora_step(s, nsteps);
+        clock_gettime(CLOCK_MONOTONIC, &b);
+        return (double)(b.tv_sec - a.tv_sec) + 1e-9 * (double)(b.tv_nsec - a.tv_nsec);
+    }
     if (s->p.model != DBA_MODEL_KH && s->p.model != DBA_MODEL_CH && s->p.model != DBA_MODEL_DENDRITE_K93)
         return -1.0;
     if (nthreads < 1) nthreads = 1;
