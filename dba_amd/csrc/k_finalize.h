@@ -345,31 +345,48 @@ __device__ __forceinline__ void finalize_body(const Inst* __restrict__ insts, co
     __syncthreads();
     if (!s_last) return;
     __threadfence();
-    for (int i = tid; i < n_inst; i += NT) {
+    // one wave per instance: lanes stride over its segments, then a fixed-shape
+    // shuffle tree (the same order on every run, and for a one-segment instance
+    // exactly that segment's partial)
+    for (int i0 = 0; i0 < n_inst; i0 += NT / 64) {
+        const int i = i0 + wid;
+        if (i >= n_inst) break;
         CInst& J = ((CInst*)insts)[i];
         const InstSegs r = isegs[i];
         int64_t ll = 0;
         unsigned long long q0 = 0, q1 = 0, qc = 0;
         double cs[kMaxFields] = {0.0, 0.0, 0.0, 0.0};
         double vm = 0.0;
-        for (int q = r.first; q < r.first + r.count; q++) {
-            ll += __hip_atomic_load(&counts[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int ncs = J.ncons;
+        for (int q = r.first + lane; q < r.first + r.count; q += 64) {
+            ll += counts[q];
             if (rotate) {
                 q0 += parts[q].c0;
                 q1 += parts[q].c1;
                 qc += parts[q].cells;
-                for (int f = 0; f < J.ncons; f++) cs[f] += parts[q].csum[f];
+                for (int f = 0; f < ncs; f++) cs[f] += parts[q].csum[f];
                 vm = fmax(vm, parts[q].vmax);
             }
         }
-        double sc[16];
-        for (int kk = 0; kk < 16; kk++) sc[kk] = J.scal[kk];
-        finish_instance(J, ll, q0, q1, qc, cs, vm, J.ctr[0], J.ctr[1], sc, rotate);
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            ll += __shfl_down(ll, off, 64);
+            q0 += __shfl_down(q0, off, 64);
+            q1 += __shfl_down(q1, off, 64);
+            qc += __shfl_down(qc, off, 64);
+            for (int f = 0; f < kMaxFields; f++) cs[f] += __shfl_down(cs[f], off, 64);
+            vm = fmax(vm, __shfl_down(vm, off, 64));
+        }
+        if (lane == 0) {
+            double sc[16];
+            for (int kk = 0; kk < 16; kk++) sc[kk] = J.scal[kk];
+            finish_instance(J, ll, q0, q1, qc, cs, vm, J.ctr[0], J.ctr[1], sc, rotate);
+        }
     }
     int running = 0;
     for (int b = 0; b < nseg; b += NT) {
         const int i = b + tid;
-        const int v = i < nseg ? __hip_atomic_load(&counts[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+        const int v = i < nseg ? counts[i] : 0;
         int tot;
         const int ex = block_exclusive_scan<NT>(v, si, &tot);
         if (i < nseg) offsets[i] = running + ex;
