@@ -60,6 +60,7 @@ struct Inst {
     double ntotal;               // number of cells (conserved mean)
     // derived constants, computed on the host with the same expressions as the oracle
     double invdx2, inv6dx2, inv2dx, dt_tau, alpha_pi, cos_jt0, sin_jt0, k_epsp;
+    double mb_cx, mb_cy, mb_cz, mb_lk, mb_omz;  // Model B: -1 + 1/r_x.., log(k_eq)/r_z, 1 - z_crit
     // model constants
     double tau, eps_bar, delta, alpha, gamma, theta0, latent_k, t_eq;
     double mobility, kappa, react_gamma, cbar;
@@ -370,6 +371,46 @@ __device__ __forceinline__ bool work_next(const WorkList& w, WorkCursor& c, int 
         return next_item(w, item, total, I, t, rec, want_rec);
     }
     return false;
+}
+
+// The blocks one workgroup visits in 64 consecutive grid-stride rounds of the
+// deal work_next makes, resolved at once (lane j: round round0 + j), so a kernel
+// can read blocks ahead without a dependent worklist load per block.
+struct RoundList {
+    uint64_t valid;  // bit j: this workgroup has a block in round round0 + j
+    int inst;        // lane j: that block's instance
+    int t;           // lane j: that block
+};
+
+__device__ __forceinline__ RoundList round_list(const WorkList& w, int total, int round0) {
+    const int G = gridDim.x, lane = threadIdx.x & 63;
+    const int64_t base = (int64_t)(round0 + lane) * G;
+    int item = -1;
+    if (base < total) item = xcd_deal(blockIdx.x, G, (int)base, (int)min((int64_t)G, (int64_t)total - base));
+    RoundList L;
+    L.inst = 0;
+    L.t = 0;
+    if (item >= 0) {
+        if (w.solo_nt > 0) {
+            L.t = w.solo_seg[item];
+        } else {
+            const int s = w.nseg == 1 ? 0 : find_slot(w.offsets, w.nseg, item);
+            const Seg sg = w.segs[s];
+            L.inst = sg.inst;
+            L.t = w.insts[sg.inst].seg[sg.tile0 + (item - w.offsets[s])];
+        }
+    }
+    L.valid = __ballot(item >= 0);
+    return L;
+}
+
+// Pops the earliest remaining round of `todo` (uniform) and returns its block.
+__device__ __forceinline__ void round_take(const WorkList& w, const RoundList& L, uint64_t& todo, CInst** I,
+                                           int* t) {
+    const int r = __builtin_ctzll(todo);
+    todo &= todo - 1;
+    *I = (CInst*)w.insts + __builtin_amdgcn_readlane(L.inst, r);
+    *t = __builtin_amdgcn_readlane(L.t, r);
 }
 
 }  // namespace dba

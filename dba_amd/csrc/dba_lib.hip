@@ -151,6 +151,13 @@ void derive_constants(const dba_params& p, Inst& h) {
     h.cos_jt0 = std::cos(jf * p.theta0);
     h.sin_jt0 = std::sin(jf * p.theta0);
     h.k_epsp = -(p.eps_bar * jf * p.delta);
+    if (p.r_x != 0.0 && p.r_y != 0.0 && p.r_z != 0.0) {  // Model B (validated there)
+        h.mb_cx = -1.0 + 1.0 / p.r_x;
+        h.mb_cy = -1.0 + 1.0 / p.r_y;
+        h.mb_cz = -1.0 + 1.0 / p.r_z;
+        h.mb_lk = std::log(p.k_eq) / p.r_z;
+    }
+    h.mb_omz = 1.0 - p.z_crit;
 }
 
 dba_status validate(const dba_params& p) {
@@ -227,10 +234,10 @@ Layout layout_of(const dba_params& p) {
     L.nf = nfields_of(p.model);
     L.nsets = nsets_of(p.model);
     L.naux = 0;
+    if (p.model == DBA_MODEL_MODELB) L.naux = 3;  // chemical potentials (every cell, every mode)
     if (p.mode != DBA_MODE_REGULAR) {
         if (p.model == DBA_MODEL_CH) L.naux = 1;   // mu ring
         if (p.model == DBA_MODEL_KH) L.naux = 24;  // edge state of sleeping tiles
-        if (p.model == DBA_MODEL_MODELB) L.naux = 3;  // chemical potentials of block surfaces
     }
     return L;
 }
@@ -474,10 +481,13 @@ void launch_step(const Group* g, hipStream_t s) {
     if (g->model == DBA_MODEL_DENDRITE_K94 || g->model == DBA_MODEL_MODELB) {
         dim3 grid(g->grid), block(64);
         if (g->model == DBA_MODEL_MODELB) {
-            if (g->tile_z == 4)
-                hipLaunchKernelGGL((modelb_step_kernel<4, 4>), grid, block, 0, s, wl_of(g));
-            else
-                hipLaunchKernelGGL((modelb_step_kernel<4, 1>), grid, block, 0, s, wl_of(g));
+            if (g->tile_z == 4) {
+                hipLaunchKernelGGL((modelb_mu_kernel<4, 4>), grid, block, 0, s, wl_of(g));
+                hipLaunchKernelGGL((modelb_update_kernel<4, 4>), grid, block, 0, s, wl_of(g));
+            } else {
+                hipLaunchKernelGGL((modelb_mu_kernel<4, 1>), grid, block, 0, s, wl_of(g));
+                hipLaunchKernelGGL((modelb_update_kernel<4, 1>), grid, block, 0, s, wl_of(g));
+            }
         } else {
             if (g->tile_z == 4)
                 hipLaunchKernelGGL((k94_step_kernel<4, 4>), grid, block, 0, s, wl_of(g));

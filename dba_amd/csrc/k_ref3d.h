@@ -5,13 +5,14 @@
 //    of Wakeup_Next (example_Dendritic_growth/kernel.cu:181-293, 322-574):
 //    split p / T sets; regular T = T + dt lapT + p_new - p, DBA T = (T + dt lapT)
 //    + (p_new - p) only where p-awake; Pi = 3.14159 (kernel.cu:7).
-//  * Ternary Flory-Huggins Model B with reaction: Calc_mu(_wakeup) + Update(_wakeup)
-//    + copy-back + detector |lap z| > eps (example_Reactive_ModelB/kernel.cu:290-763),
-//    conserved change capture (example_Reactive_ModelB/Conserved/kernel.cu:630-702).
-//    The chemical potentials of sleeping neighbours are the ones they stored
-//    when they last ran (a block stores its surface cells).
+//  * Ternary Flory-Huggins Model B with reaction: Calc_mu(_wakeup) + detector
+//    |lap z| > eps, then Update(_wakeup) + copy-back (example_Reactive_ModelB/
+//    kernel.cu:290-763), conserved change capture (example_Reactive_ModelB/
+//    Conserved/kernel.cu:630-702). The chemical potentials of sleeping
+//    neighbours are the ones they stored when they last ran.
 #pragma once
 #include "common.h"
+#include "mathf64.h"
 
 namespace dba {
 
@@ -73,126 +74,351 @@ __device__ __forceinline__ int nb_dir3(int lx, int ly, int lz) {
 }
 
 // ------------------------------------------------------------ Model B (ternary)
+// Two launches per step, like the reference's Calc_mu / Update pair: the first
+// computes the chemical potentials of every awake block's own cells into the
+// stored-mu arrays (aux 0..2), the second updates x, y, z from the potentials
+// of the block and its 1-cell ring. Sleeping neighbours keep the potentials they
+// stored when they last ran, so the ring needs no recomputation and each cell's
+// logarithms are taken once per step.
+//
+// One wave per 4x4x4 block, one cell per lane. Both kernels are software
+// pipelined over the workgroup's blocks (resolved 64 rounds at a time by
+// round_list): while block i computes from LDS, block i+1's box is in flight
+// into registers and block i+2's parity words are in flight, so no block waits
+// on a dependent chain of loads (worklist -> parities -> box).
+
+// The halo box elements a lane loads (k = lane + 64 i), packed as
+// lx | ly << 4 | lz << 8 | neighbour direction << 12 (the same for every block).
 template <int TB, int TZ>
-__global__ __launch_bounds__(64) void modelb_step_kernel(const WorkList wl) {
-    constexpr int NT = 64;
-    constexpr int W2X = TB + 4, W2Z = TZ + 4;  // x, y, z with a 2-cell halo
-    constexpr int W1X = TB + 2, W1Z = TZ + 2;  // mu with a 1-cell halo
-    constexpr int A2 = W2X * W2X * W2Z;
-    constexpr int A1 = W1X * W1X * W1Z;
-    __shared__ double sc[3 * A2];
-    __shared__ double smu[3 * A1];
-    __shared__ double sred[2];
-    __shared__ int sflag[2];
-    __shared__ NbTable3 nb;
-
-    WorkCursor cur;
-    work_begin(wl, cur, false);
-    const int total = wl.offsets[wl.nseg];
-    const int tid = threadIdx.x;
-    for (;;) {
-        CInst* Ip;
-        int t;
-        uint64_t rec;
-        if (!work_next(wl, cur, total, &Ip, &t, &rec, false)) break;
-        CInst& I = *Ip;
-        const int nx = I.nx, ny = I.ny, nz = I.nz;
-        const int x0 = (t % I.ntx) * TB, y0 = ((t / I.ntx) % I.nty) * TB, z0 = (t / (I.ntx * I.nty)) * TZ;
-        const bool regular = I.mode == 0;
-        const int ncons = I.ncons;
-        const int64_t nxy = (int64_t)nx * ny;
-        nb_fill3<TB, TZ>(nb, I, x0, y0, z0, regular, 1, ncons);
-        __syncthreads();
-        const int pself = nb.par[0][13];
-
-        // x, y, z on the block + 2-cell halo
-        for (int k = tid; k < A2; k += NT) {
-            const int lx = k % W2X, ly = (k / W2X) % W2X, lz = k / (W2X * W2X);
-            const int d = nb_dir3<TB, TZ, 2>(lx, ly, lz);
-            const int64_t gi = wrapi(x0 + lx - 2, nx) + wrapi(y0 + ly - 2, ny) * (int64_t)nx +
-                               wrapi(z0 + lz - 2, nz) * nxy;
-            const int pp = nb.par[0][d];
-            for (int f = 0; f < 3; f++) sc[f * A2 + k] = I.buf[pp][f][gi] - nb.shift[f][d];
+struct Box1 {
+    static constexpr int WX = TB + 2, WZ = TZ + 2, A = WX * WX * WZ, NH = (A + 63) / 64;
+    int c[NH];
+    __device__ __forceinline__ void init(int lane) {
+#pragma unroll
+        for (int i = 0; i < NH; i++) {
+            const int k = lane + i * 64;
+            const int lx = k % WX, ly = (k / WX) % WX, lz = k / (WX * WX);
+            c[i] = lx | ly << 4 | lz << 8 | nb_dir3<TB, TZ, 1>(lx, ly, lz) << 12;
         }
-        __syncthreads();
+    }
+};
 
-        // chemical potentials on the block + 1-cell ring (Calc_mu_wakeup)
-        int flag = 0;
-        const double inv_dx2 = 1.0 / (I.dx * I.dx);
-        for (int k = tid; k < A1; k += NT) {
-            const int lx = k % W1X, ly = (k / W1X) % W1X, lz = k / (W1X * W1X);
-            const int d = nb_dir3<TB, TZ, 1>(lx, ly, lz);
-            const int cx = lx + 1, cy = ly + 1, cz = lz + 1;
-            const int64_t gi = wrapi(x0 + lx - 1, nx) + wrapi(y0 + ly - 1, ny) * (int64_t)nx +
-                               wrapi(z0 + lz - 1, nz) * nxy;
-            if (!nb.act[0][d]) {
-                for (int f = 0; f < 3; f++) smu[f * A1 + k] = I.aux[f][gi];
-                continue;
+// v in [-1, n] wrapped into [0, n)
+__device__ __forceinline__ int wrap1(int v, int n) {
+    v = v < 0 ? v + n : v;
+    return v >= n ? v - n : v;
+}
+
+// A block of the worklist and what its loads need (all wave-uniform).
+struct MbBlock {
+    CInst* I;
+    int t, x0, y0, z0;
+    uint64_t pm;  // parity of the 27 neighbour blocks, bit d = (dx+1) + 3 (dy+1) + 9 (dz+1)
+};
+
+template <int TB, int TZ>
+__device__ __forceinline__ MbBlock mb_take(const WorkList& w, const RoundList& L, uint64_t& todo) {
+    MbBlock b;
+    round_take(w, L, todo, &b.I, &b.t);
+    CInst& I = *b.I;
+    b.x0 = (b.t % I.ntx) * TB;
+    b.y0 = ((b.t / I.ntx) % I.nty) * TB;
+    b.z0 = (b.t / (I.ntx * I.nty)) * TZ;
+    b.pm = 0;
+    return b;
+}
+
+// Parity of neighbour block `lane` (lanes 0..26): the bitmap word is loaded
+// here and the bit taken later (mb_parities_finish), so the load stays in flight.
+struct ParityWord {
+    uint64_t word;
+    int tt;
+};
+
+template <int TB, int TZ>
+__device__ __forceinline__ ParityWord mb_parities_issue(const MbBlock& b, int lane) {
+    CInst& I = *b.I;
+    ParityWord p{0ull, 0};
+    if (lane < 27) {
+        const int dx = lane % 3 - 1, dy = (lane / 3) % 3 - 1, dz = lane / 9 - 1;
+        const int mx = wrap1(dx < 0 ? b.x0 - 1 : b.x0 + dx * TB, I.nx);
+        const int my = wrap1(dy < 0 ? b.y0 - 1 : b.y0 + dy * TB, I.ny);
+        const int mz = wrap1(dz < 0 ? b.z0 - 1 : b.z0 + dz * TZ, I.nz);
+        p.tt = mx / TB + (my / TB) * I.ntx + (mz / TZ) * I.ntx * I.nty;
+        p.word = I.par[0][p.tt >> 6];
+    }
+    return p;
+}
+
+// One ballot gathers the 27 bits; with conserved DBA lanes 0..26 also store each
+// neighbour's pending shift S_f - epoch in sshift (read after a barrier).
+__device__ __forceinline__ void mb_parities_finish(MbBlock& b, const ParityWord& p, int lane,
+                                                   double (*sshift)[27]) {
+    CInst& I = *b.I;
+    const int bit = lane < 27 ? (int)((p.word >> (p.tt & 63)) & 1ull) : 0;
+    b.pm = __ballot(bit);
+    if (lane < 27)
+        for (int f = 0; f < I.ncons; f++)
+            sshift[f][lane] = I.scal[8 + f] - I.s_tile[((int64_t)f * 2 + bit) * I.nt + p.tt];
+}
+
+template <int TB, int TZ>
+__device__ __forceinline__ int64_t mb_box_index(const MbBlock& b, int c) {
+    CInst& I = *b.I;
+    const int lx = c & 15, ly = (c >> 4) & 15, lz = (c >> 8) & 15;
+    return wrap1(b.x0 + lx - 1, I.nx) + (int64_t)wrap1(b.y0 + ly - 1, I.ny) * I.nx +
+           (int64_t)wrap1(b.z0 + lz - 1, I.nz) * ((int64_t)I.nx * I.ny);
+}
+
+template <int TB, int TZ>
+__device__ __forceinline__ int64_t mb_own_index(const MbBlock& b, int lane) {
+    CInst& I = *b.I;
+    const int lx = lane % TB, ly = (lane / TB) % TB, lz = lane / (TB * TB);
+    return (b.x0 + lx) + (int64_t)(b.y0 + ly) * I.nx + (int64_t)(b.z0 + lz) * ((int64_t)I.nx * I.ny);
+}
+
+// Calc_mu(_wakeup) + the detector |lap z| > eps (example_Reactive_ModelB/kernel.cu:290-420)
+template <int TB, int TZ>
+__global__ __launch_bounds__(64) void modelb_mu_kernel(const WorkList wl) {
+    using B1 = Box1<TB, TZ>;
+    constexpr int WX = B1::WX, A = B1::A, NH = B1::NH, C = TB * TB * TZ;
+    static_assert(C <= 64, "at most one cell per lane");
+    __shared__ double sc[3 * A];
+    __shared__ double sshift[2][kMaxFields][27];
+
+    const int lane = threadIdx.x;
+    B1 box;
+    box.init(lane);
+    double h[NH][3];
+    // block q's box (parities known) into h, its own phi into ph
+    auto load = [&](const MbBlock& q, double& ph) {
+        CInst& I = *q.I;
+#pragma unroll
+        for (int i = 0; i < NH; i++) {
+            if (lane + i * 64 < A) {
+                const int c = box.c[i];
+                const int64_t gi = mb_box_index<TB, TZ>(q, c);
+                const int pp = (int)(q.pm >> (c >> 12)) & 1;
+                for (int f = 0; f < 3; f++) h[i][f] = I.buf[pp][f][gi];
             }
-            const int q = cx + cy * W2X + cz * W2X * W2X;
-            const double X = sc[q], Y = sc[A2 + q], Z = sc[2 * A2 + q];
-            const double phi = I.buf[pself][3][gi];
-            const double B = 1.0 - X - Y - Z;
-            const double lx_ = lds_lap27<W2X, W2X>(sc, cx, cy, cz, inv_dx2);
-            const double ly_ = lds_lap27<W2X, W2X>(sc + A2, cx, cy, cz, inv_dx2);
-            const double lz_ = lds_lap27<W2X, W2X>(sc + 2 * A2, cx, cy, cz, inv_dx2);
-            const double dfx = -1.0 + 1.0 / I.r_x + I.chi_xy * Y + I.chi_xz * Z + I.chi_xb * B - I.chi_xb * X -
-                               I.chi_yb * Y - I.chi_zb * Z + log(X) / I.r_x - log(B) - I.epsx_sq * lx_;
-            const double dfy = -1.0 + 1.0 / I.r_y + I.chi_xy * X + I.chi_yz * Z + I.chi_yb * B - I.chi_xb * X -
-                               I.chi_yb * Y - I.chi_zb * Z + log(Y) / I.r_y - log(B) - I.epsy_sq * ly_;
-            const double dfz = -1.0 + 1.0 / I.r_z + I.chi_xz * X + I.chi_yz * Y + I.chi_zb * B - I.chi_xb * X -
-                               I.chi_yb * Y - I.chi_zb * Z + log(Z) / I.r_z - log(B) -
-                               I.p_gel * phi * phi / 2.0 / (1.0 - I.z_crit) - I.epsz_sq * lz_ -
-                               log(I.k_eq) / I.r_z;
-            smu[k] = dfx;
-            smu[A1 + k] = dfy;
-            smu[2 * A1 + k] = dfz;
-            if (d == 13) {
-                flag |= fabs(lz_) > I.thresh_a;
-                if (!regular) {
-                    I.aux[0][gi] = dfx;
-                    I.aux[1][gi] = dfy;
-                    I.aux[2][gi] = dfz;
+        }
+        if (lane < C) ph = I.buf[(int)(q.pm >> 13) & 1][3][mb_own_index<TB, TZ>(q, lane)];
+    };
+
+    const int total = wl.offsets[wl.nseg];
+    for (int round0 = 0; (int64_t)round0 * gridDim.x < total; round0 += 64) {
+        const RoundList L = round_list(wl, total, round0);
+        uint64_t todo = L.valid;
+        if (!todo) continue;
+        MbBlock b = mb_take<TB, TZ>(wl, L, todo);
+        mb_parities_finish(b, mb_parities_issue<TB, TZ>(b, lane), lane, sshift[0]);
+        __syncthreads();
+        double phi = 0.0;
+        load(b, phi);
+        bool has1 = todo != 0;
+        MbBlock n1{};
+        ParityWord w1{0ull, 0};
+        if (has1) {
+            n1 = mb_take<TB, TZ>(wl, L, todo);
+            w1 = mb_parities_issue<TB, TZ>(n1, lane);
+        }
+        int slot = 0;
+        for (;;) {
+            CInst& I = *b.I;
+            const int ncons = I.ncons;
+            // block b's box into LDS, each value in the current frame
+#pragma unroll
+            for (int i = 0; i < NH; i++) {
+                const int k = lane + i * 64;
+                if (k < A) {
+                    const int d = box.c[i] >> 12;
+                    for (int f = 0; f < 3; f++) {
+                        double v = h[i][f];
+                        if (f < ncons) v -= sshift[slot][f][d];
+                        sc[f * A + k] = v;
+                    }
                 }
             }
-        }
-        __syncthreads();
+            __syncthreads();
+            double phi_n = 0.0;
+            bool has2 = false;
+            MbBlock n2{};
+            ParityWord w2{0ull, 0};
+            if (has1) {
+                mb_parities_finish(n1, w1, lane, sshift[slot ^ 1]);
+                load(n1, phi_n);
+                if (todo) {
+                    n2 = mb_take<TB, TZ>(wl, L, todo);
+                    w2 = mb_parities_issue<TB, TZ>(n2, lane);
+                    has2 = true;
+                }
+            }
 
-        // Update_wakeup on the block
-        double part[3] = {0.0, 0.0, 0.0};
-        for (int k = tid; k < TB * TB * TZ; k += NT) {
-            const int lx = k % TB, ly = (k / TB) % TB, lz = k / (TB * TB);
-            const int64_t gi = (x0 + lx) + (int64_t)(y0 + ly) * nx + (int64_t)(z0 + lz) * nxy;
-            const int mq = (lx + 1) + (ly + 1) * W1X + (lz + 1) * W1X * W1X;
-            const int cq = (lx + 2) + (ly + 2) * W2X + (lz + 2) * W2X * W2X;
-            const double dfx = smu[mq], dfy = smu[A1 + mq], dfz = smu[2 * A1 + mq];
-            const double R = I.k_0 * (exp(I.n_exp * I.v_x * dfx + I.m_exp * I.v_y * dfy) - exp(I.v_z * dfz));
-            const double chx = I.dt * I.v_x *
-                               (I.mob_x * lds_lap27<W1X, W1X>(smu, lx + 1, ly + 1, lz + 1, inv_dx2) - I.n_exp * R);
-            const double chy = I.dt * I.v_y *
-                               (I.mob_y * lds_lap27<W1X, W1X>(smu + A1, lx + 1, ly + 1, lz + 1, inv_dx2) -
-                                I.m_exp * R);
-            const double chz = I.dt * I.v_z *
-                               (I.mob_z * lds_lap27<W1X, W1X>(smu + 2 * A1, lx + 1, ly + 1, lz + 1, inv_dx2) + R);
-            I.buf[pself ^ 1][0][gi] = sc[cq] + chx;
-            I.buf[pself ^ 1][1][gi] = sc[A2 + cq] + chy;
-            I.buf[pself ^ 1][2][gi] = sc[2 * A2 + cq] + chz;
-            part[0] += chx;
-            part[1] += chy;
-            part[2] += chz;
+            int flag = 0;
+            if (lane < C) {
+                const int lx = lane % TB, ly = (lane / TB) % TB, lz = lane / (TB * TB);
+                const int cx = lx + 1, cy = ly + 1, cz = lz + 1;
+                const int q = cx + cy * WX + cz * WX * WX;
+                const double X = sc[q], Y = sc[A + q], Z = sc[2 * A + q];
+                const double B = 1.0 - X - Y - Z;
+                // logarithms first, then one Laplacian at a time: left free, the
+                // scheduler hoists all 81 LDS reads above the logarithms (250 VGPRs)
+                const double logB = log_f64(B), logX = log_f64(X), logY = log_f64(Y), logZ = log_f64(Z);
+                __builtin_amdgcn_sched_barrier(0);
+                const double inv_dx2 = I.invdx2;
+                const double lx_ = lds_lap27<WX, WX>(sc, cx, cy, cz, inv_dx2);
+                __builtin_amdgcn_sched_barrier(0);
+                const double ly_ = lds_lap27<WX, WX>(sc + A, cx, cy, cz, inv_dx2);
+                __builtin_amdgcn_sched_barrier(0);
+                const double lz_ = lds_lap27<WX, WX>(sc + 2 * A, cx, cy, cz, inv_dx2);
+                const double dfx = I.mb_cx + I.chi_xy * Y + I.chi_xz * Z + I.chi_xb * B - I.chi_xb * X -
+                                   I.chi_yb * Y - I.chi_zb * Z + logX / I.r_x - logB - I.epsx_sq * lx_;
+                const double dfy = I.mb_cy + I.chi_xy * X + I.chi_yz * Z + I.chi_yb * B - I.chi_xb * X -
+                                   I.chi_yb * Y - I.chi_zb * Z + logY / I.r_y - logB - I.epsy_sq * ly_;
+                const double dfz = I.mb_cz + I.chi_xz * X + I.chi_yz * Y + I.chi_zb * B - I.chi_xb * X -
+                                   I.chi_yb * Y - I.chi_zb * Z + logZ / I.r_z - logB -
+                                   I.p_gel * phi * phi / 2.0 / I.mb_omz - I.epsz_sq * lz_ - I.mb_lk;
+                const int64_t gi = mb_own_index<TB, TZ>(b, lane);
+                I.aux[0][gi] = dfx;
+                I.aux[1][gi] = dfy;
+                I.aux[2][gi] = dfz;
+                flag = fabs(lz_) > I.thresh_a;
+            }
+            if (I.mode != 0) {
+                const int any = __ballot(flag) != 0ull;
+                if (lane == 0) I.nflag[0][b.t] = (uint8_t)any;
+            }
+            __syncthreads();
+            if (!has1) break;
+            b = n1;
+            phi = phi_n;
+            n1 = n2;
+            w1 = w2;
+            has1 = has2;
+            slot ^= 1;
         }
-        if (!regular) {
-            const int any = block_any<NT>(flag, sflag);
-            if (tid == 0) I.nflag[0][t] = (uint8_t)any;
-        }
-        for (int f = 0; f < ncons; f++) {
-            const double tot = block_sum<NT>(part[f], sred);
-            if (tid == 0) {
-                I.tpart[(int64_t)f * I.nt + t] = tot;
-                I.s_tile[((int64_t)f * 2 + (pself ^ 1)) * I.nt + t] = I.scal[8 + f];
+    }
+}
+
+// Update(_wakeup) + copy-back + conserved change capture
+// (example_Reactive_ModelB/kernel.cu:430-560, Conserved/kernel.cu:630-702)
+template <int TB, int TZ>
+__global__ __launch_bounds__(64) void modelb_update_kernel(const WorkList wl) {
+    using B1 = Box1<TB, TZ>;
+    constexpr int WX = B1::WX, A = B1::A, NH = B1::NH, C = TB * TB * TZ;
+    static_assert(C <= 64, "at most one cell per lane");
+    __shared__ double smu[3 * A];
+
+    const int lane = threadIdx.x;
+    B1 box;
+    box.init(lane);
+    double h[NH][3];
+    // block q's potentials box into h, its own x, y, z (buffer pq) into vq and
+    // their shift epochs into eq (conserved)
+    auto load = [&](const MbBlock& q, int pq, double (&vq)[3], double (&eq)[3]) {
+        CInst& I = *q.I;
+#pragma unroll
+        for (int i = 0; i < NH; i++) {
+            if (lane + i * 64 < A) {
+                const int64_t gi = mb_box_index<TB, TZ>(q, box.c[i]);
+                for (int f = 0; f < 3; f++) h[i][f] = I.aux[f][gi];
             }
         }
-        __syncthreads();
+        if (lane < C) {
+            const int64_t gi = mb_own_index<TB, TZ>(q, lane);
+            for (int f = 0; f < 3; f++) vq[f] = I.buf[pq][f][gi];
+        }
+        for (int f = 0; f < I.ncons; f++) eq[f] = I.s_tile[((int64_t)f * 2 + pq) * I.nt + q.t];
+    };
+
+    const int total = wl.offsets[wl.nseg];
+    for (int round0 = 0; (int64_t)round0 * gridDim.x < total; round0 += 64) {
+        const RoundList L = round_list(wl, total, round0);
+        uint64_t todo = L.valid;
+        if (!todo) continue;
+        MbBlock b = mb_take<TB, TZ>(wl, L, todo);
+        int pself = bit_of(b.I->par[0], b.t);
+        double v[3] = {0.0, 0.0, 0.0}, e[3] = {0.0, 0.0, 0.0};
+        load(b, pself, v, e);
+        bool has1 = todo != 0;
+        MbBlock n1{};
+        uint64_t w1 = 0;
+        if (has1) {
+            n1 = mb_take<TB, TZ>(wl, L, todo);
+            w1 = n1.I->par[0][n1.t >> 6];
+        }
+        for (;;) {
+#pragma unroll
+            for (int i = 0; i < NH; i++) {
+                const int k = lane + i * 64;
+                if (k < A)
+                    for (int f = 0; f < 3; f++) smu[f * A + k] = h[i][f];
+            }
+            __syncthreads();
+            double vn[3] = {0.0, 0.0, 0.0}, en[3] = {0.0, 0.0, 0.0};
+            int pn = 0;
+            bool has2 = false;
+            MbBlock n2{};
+            uint64_t w2 = 0;
+            if (has1) {
+                pn = (int)((w1 >> (n1.t & 63)) & 1ull);
+                load(n1, pn, vn, en);
+                if (todo) {
+                    n2 = mb_take<TB, TZ>(wl, L, todo);
+                    w2 = n2.I->par[0][n2.t >> 6];
+                    has2 = true;
+                }
+            }
+
+            CInst& I = *b.I;
+            const int ncons = I.ncons;
+            double part[3] = {0.0, 0.0, 0.0};
+            if (lane < C) {
+                const int lx = lane % TB, ly = (lane / TB) % TB, lz = lane / (TB * TB);
+                double x[3] = {v[0], v[1], v[2]};
+                for (int f = 0; f < ncons; f++) x[f] -= I.scal[8 + f] - e[f];
+                const double inv_dx2 = I.invdx2;
+                const int mq = (lx + 1) + (ly + 1) * WX + (lz + 1) * WX * WX;
+                const double dfx = smu[mq], dfy = smu[A + mq], dfz = smu[2 * A + mq];
+                const double R = I.k_0 * (exp(I.n_exp * I.v_x * dfx + I.m_exp * I.v_y * dfy) - exp(I.v_z * dfz));
+                __builtin_amdgcn_sched_barrier(0);
+                const double chx =
+                    I.dt * I.v_x * (I.mob_x * lds_lap27<WX, WX>(smu, lx + 1, ly + 1, lz + 1, inv_dx2) - I.n_exp * R);
+                __builtin_amdgcn_sched_barrier(0);
+                const double chy = I.dt * I.v_y *
+                                   (I.mob_y * lds_lap27<WX, WX>(smu + A, lx + 1, ly + 1, lz + 1, inv_dx2) - I.m_exp * R);
+                __builtin_amdgcn_sched_barrier(0);
+                const double chz =
+                    I.dt * I.v_z * (I.mob_z * lds_lap27<WX, WX>(smu + 2 * A, lx + 1, ly + 1, lz + 1, inv_dx2) + R);
+                const int64_t gi = mb_own_index<TB, TZ>(b, lane);
+                I.buf[pself ^ 1][0][gi] = x[0] + chx;
+                I.buf[pself ^ 1][1][gi] = x[1] + chy;
+                I.buf[pself ^ 1][2][gi] = x[2] + chz;
+                part[0] = chx;
+                part[1] = chy;
+                part[2] = chz;
+            }
+            for (int f = 0; f < ncons; f++) {
+                double tot = part[f];
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) tot += __shfl_down(tot, off, 64);
+                if (lane == 0) {
+                    I.tpart[(int64_t)f * I.nt + b.t] = tot;
+                    I.s_tile[((int64_t)f * 2 + (pself ^ 1)) * I.nt + b.t] = I.scal[8 + f];
+                }
+            }
+            __syncthreads();
+            if (!has1) break;
+            b = n1;
+            pself = pn;
+            for (int f = 0; f < 3; f++) {
+                v[f] = vn[f];
+                e[f] = en[f];
+            }
+            n1 = n2;
+            w1 = w2;
+            has1 = has2;
+        }
     }
 }
 
