@@ -133,6 +133,8 @@ SIGNATURES = [
     ("dba_field_sum", C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
     ("dba_growth_front", C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p]),
     ("dba_kh_vy_mode", C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+    ("dba_droplets", C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64),
+                                 C.POINTER(C.c_int64), C.c_void_p, C.c_int64]),
 ]
 
 _lib = None

@@ -26,6 +26,7 @@
 #include "k_kh.h"
 #include "k_ref3d.h"
 #include "k_diag.h"
+#include "k_droplets.h"
 
 using namespace dba;
 
@@ -620,6 +621,24 @@ dba_status ensure_staging(dba_ctx* ctx, size_t n) {
 
 // ===================================================================== C ABI
 
+namespace {
+// Device scratch of one droplet analysis, freed when the call returns.
+struct CcScratch {
+    std::vector<void*> ptrs;
+    ~CcScratch() {
+        for (void* q : ptrs) (void)hipFree(q);
+    }
+    template <typename T>
+    hipError_t get(T** out, size_t count) {
+        void* q = nullptr;
+        const hipError_t e = hipMalloc(&q, sizeof(T) * (count ? count : 1));
+        if (e == hipSuccess) ptrs.push_back(q);
+        *out = (T*)q;
+        return e;
+    }
+};
+}  // namespace
+
 extern "C" {
 
 const char* dba_last_error(void) { return g_error.c_str(); }
@@ -1051,6 +1070,59 @@ dba_status dba_growth_front(dba_ctx* ctx, int32_t id, const int32_t* xs, int32_t
     LAUNCH_CHECK();
     HIP_TRY(hipMemcpyAsync(front, res, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return DBA_OK;
+}
+
+dba_status dba_droplets(dba_ctx* ctx, int32_t id, int32_t field, double* avg, int64_t* volume,
+                        int64_t* n_droplets, int64_t* sizes, int64_t cap) {
+    if (!ctx || !avg || !volume || !n_droplets) return fail(DBA_ERR_INVALID, "null argument");
+    if (cap < 0 || (cap > 0 && !sizes)) return fail(DBA_ERR_INVALID, "sizes must hold cap >= 0 entries");
+    Instance* in = find(ctx, id);
+    if (!in) return fail(DBA_ERR_NOTFOUND, "unknown instance %d", id);
+    if (field < 0 || field >= in->h.nfields) return fail(DBA_ERR_INVALID, "field %d invalid", field);
+    const int64_t n = (int64_t)in->p.nx * in->p.ny * in->p.nz;
+    if (n >= INT32_MAX) return fail(DBA_ERR_INVALID, "droplet analysis supports grids below 2^31 cells");
+    CcScratch sc;
+    uint8_t* cls;
+    int32_t *parent, *size, *key;
+    double *part, *d_avg;
+    int64_t *vpart, *counters;
+    if (sc.get(&cls, n) != hipSuccess || sc.get(&parent, n) != hipSuccess || sc.get(&size, n) != hipSuccess ||
+        sc.get(&key, n) != hipSuccess || sc.get(&part, 2 * kCcBlocks) != hipSuccess || sc.get(&d_avg, 1) != hipSuccess ||
+        sc.get(&vpart, kCcBlocks) != hipSuccess || sc.get(&counters, 2) != hipSuccess)
+        return fail(DBA_ERR_NOMEM, "droplet analysis: no device memory for %lld cells", (long long)n);
+    hipStream_t st = ctx->stream;
+    const dim3 g(kCcBlocks), b(kCcNT);
+    hipLaunchKernelGGL(cc_minmax_kernel, g, b, 0, st, in->d_self, 0, field, part);
+    hipLaunchKernelGGL(cc_threshold_kernel, dim3(1), dim3(64), 0, st, part, d_avg);
+    hipLaunchKernelGGL(cc_init_kernel, g, b, 0, st, in->d_self, 0, field, d_avg, cls, parent, size, key);
+    hipLaunchKernelGGL(cc_union_kernel, g, b, 0, st, cls, parent, in->p.nx, in->p.ny, in->p.nz);
+    hipLaunchKernelGGL(cc_count_kernel, g, b, 0, st, cls, parent, size, key, n, vpart);
+    HIP_TRY(hipMemsetAsync(counters, 0, 2 * sizeof(int64_t), st));
+    hipLaunchKernelGGL(cc_collect_kernel, g, b, 0, st, size, key, n, vpart, counters, (int64_t*)nullptr,
+                       (int64_t)0);
+    LAUNCH_CHECK();
+    int64_t cnt[2];
+    HIP_TRY(hipMemcpyAsync(cnt, counters, sizeof(cnt), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(avg, d_avg, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    *n_droplets = cnt[0];
+    *volume = cnt[1];
+    if (cap == 0 || cnt[0] == 0) return DBA_OK;
+    // second pass with room for every droplet, then the reference's order:
+    // droplets by their first seed in index order
+    int64_t* pairs;
+    if (sc.get(&pairs, 2 * cnt[0]) != hipSuccess) return fail(DBA_ERR_NOMEM, "droplet analysis: no device memory");
+    HIP_TRY(hipMemsetAsync(counters, 0, sizeof(int64_t), st));
+    hipLaunchKernelGGL(cc_collect_kernel, g, b, 0, st, size, key, n, vpart, counters, pairs, cnt[0]);
+    LAUNCH_CHECK();
+    std::vector<int64_t> h((size_t)(2 * cnt[0]));
+    HIP_TRY(hipMemcpyAsync(h.data(), pairs, sizeof(int64_t) * h.size(), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    std::vector<std::pair<int64_t, int64_t>> v((size_t)cnt[0]);
+    for (int64_t k = 0; k < cnt[0]; k++) v[(size_t)k] = {h[(size_t)(2 * k)], h[(size_t)(2 * k + 1)]};
+    std::sort(v.begin(), v.end());
+    for (int64_t k = 0; k < cnt[0] && k < cap; k++) sizes[k] = v[(size_t)k].second;
     return DBA_OK;
 }
 

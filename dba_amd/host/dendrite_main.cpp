@@ -15,6 +15,7 @@
 //   vtk 1           also write p_<t>.vtk and T_<t>.vtk (STRUCTURED_POINTS, the
 //                   layout Compare_VTK and Extract_Tip read) every t_freq steps
 //   K, j, theta0, T_eq, radius, T0, bc (periodic|noflux), graph_chunk, threads
+//   dump 1          also write the raw fields p_<t>.bin, T_<t>.bin (FP64, x fastest)
 #include "host_common.h"
 
 using namespace host;
@@ -79,6 +80,7 @@ int main() {
     p.t_eq = getd(kv, "T_eq", 1.0);
     p.threads = geti(kv, "threads", 0);
     const int vtk = geti(kv, "vtk", 0);
+    const int dump = geti(kv, "dump", 0);
     const int chunk = geti(kv, "graph_chunk", 100);
 
     Sim sim(p);
@@ -115,6 +117,10 @@ int main() {
         write_structured("p", 0, g, phi.data());
         write_structured("T", 0, g, T.data());
     }
+    if (dump) {
+        write_raw("p_0.bin", phi);
+        write_raw("T_0.bin", T);
+    }
 
     Clock loop;
     double stepping_s = 0.0;
@@ -150,11 +156,17 @@ int main() {
         std::ostringstream tip;
         for (size_t k = 0; k + 1 < front.size(); k++) tip << front[k] << " ";
         append_line("tip_profile.out", "", tip.str());
-        if (vtk) {
+        if (vtk || dump) {
             phi = sim.download(DBA_F_PHI);
             T = sim.download(DBA_F_T);
+        }
+        if (vtk) {
             write_structured("p", t, g, phi.data());
             write_structured("T", t, g, T.data());
+        }
+        if (dump) {
+            write_raw("p_" + std::to_string(t) + ".bin", phi);
+            write_raw("T_" + std::to_string(t) + ".bin", T);
         }
     }
     dba_info info;

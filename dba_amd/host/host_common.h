@@ -179,6 +179,17 @@ inline void append_line(const std::string& name, const std::string& header, cons
     o << line << std::endl;
 }
 
+// Raw dump of a field for comparisons beyond the VTK text's 6 significant
+// digits: little-endian FP64, x fastest, no header.
+inline void write_raw(const std::string& name, const std::vector<double>& a) {
+    std::ofstream o(name, std::ios::binary);
+    if (!o.is_open()) {
+        std::cout << "!!!!!Can not open" << name << "!! Exit!!!!!!!!" << std::endl;
+        std::exit(1);
+    }
+    o.write(reinterpret_cast<const char*>(a.data()), (std::streamsize)(a.size() * sizeof(double)));
+}
+
 struct Clock {
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     double seconds() const {

@@ -20,6 +20,7 @@
 //   cfl C          CFL time step dt = C dx / max signal speed (default 0: fixed dt)
 //   ic reference|sharp       smooth reference profile (default) or sharp shear layers
 //   graph_chunk, threads
+//   dump 1         also write the conserved state Mass|Momx|Momy|Energy_<t>.bin (FP64)
 #include "host_common.h"
 
 using namespace host;
@@ -67,6 +68,7 @@ int main() {
     p.threads = geti(kv, "threads", 0);
     const bool sharp = gets(kv, "ic", "reference") == "sharp";
     const int chunk = geti(kv, "graph_chunk", 100);
+    const int dump = geti(kv, "dump", 0);
     const double gg = p.gas_gamma, vol = dx * dy;
 
     Sim sim(p);
@@ -126,6 +128,10 @@ int main() {
         }
     };
     auto vtk = [&](int t) {
+        if (dump) {  // the conserved state, reference array names
+            const char* names[4] = {"Mass", "Momx", "Momy", "Energy"};
+            for (int k = 0; k < 4; k++) write_raw(std::string(names[k]) + "_" + std::to_string(t) + ".bin", U[k]);
+        }
         write_rectilinear("output_" + std::to_string(mode) + "_" + std::to_string(t) + ".vtk", g,
                           {{"rho", rho.data()}, {"p", pr.data()}}, vx.data(), vy.data());
     };

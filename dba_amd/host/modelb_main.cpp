@@ -20,6 +20,11 @@
 //   seed S           splitmix64 seed of the initial noise (default 1234)
 //   ch only: mobility, kappa, react_gamma, cbar, amp, stencil (5|9)
 //   graph_chunk, threads
+//   dump 1           also write the raw fields <name>_<t>.bin (FP64, x fastest)
+//   droplets 1       droplet analysis of Z (ch: C) on the device at every output:
+//                    appends "k,R" to avg_radius.txt and the droplet sizes to
+//                    size_dist.txt, k = t / t_freq (the files and rule of
+//                    coarsening_analysis.cpp, run on output_<k t_freq>.vtk)
 #include "host_common.h"
 
 using namespace host;
@@ -109,6 +114,8 @@ int main() {
     p.stencil = geti(kv, "stencil", 9);
     const uint64_t seed = (uint64_t)geti(kv, "seed", 1234);
     const int chunk = geti(kv, "graph_chunk", 100);
+    const int dump = geti(kv, "dump", 0);
+    const int droplets = geti(kv, "droplets", 0);
 
     Sim sim(p);
     const Grid& g = sim.g;
@@ -137,7 +144,29 @@ int main() {
     }
     for (size_t k = 0; k < f.size(); k++) sim.upload((int)k, f[k]);
 
+    auto analyse = [&](int t) {  // coarsening_analysis.cpp:118-175 on the device
+        const int field = ch ? 0 : 2;
+        double avg = 0.0;
+        int64_t vol = 0, cnt = 0;
+        check(dba_droplets(sim.ctx, sim.id, field, &avg, &vol, &cnt, nullptr, 0), "dba_droplets");
+        std::vector<int64_t> sizes((size_t)cnt);
+        if (cnt) check(dba_droplets(sim.ctx, sim.id, field, &avg, &vol, &cnt, sizes.data(), cnt), "dba_droplets");
+        const double N = (double)cnt;
+        const double R = Nz > 1 ? std::cbrt(vol * 0.75 / M_PI / N) : std::sqrt(vol / M_PI / N);
+        std::ostringstream sz, rad;
+        for (int64_t v : sizes) sz << v << " ";
+        rad << t / t_freq << "," << R;
+        append_line("size_dist.txt", "", sz.str());
+        append_line("avg_radius.txt", "", rad.str());
+        std::cout << "avg = " << avg << ", volume = " << vol << ", droplets = " << cnt << ", R = " << R << std::endl;
+    };
     auto output = [&](int t) {
+        if (droplets) analyse(t);
+        if (dump) {
+            const char* names[4] = {"X", "Y", "Z", "PHI"};
+            for (size_t k = 0; k < f.size(); k++)
+                write_raw(std::string(ch ? "C" : names[k]) + "_" + std::to_string(t) + ".bin", f[k]);
+        }
         if (ch) {
             double s = 0.0;
             for (size_t i = 0; i < n; i++) s += f[0][i];

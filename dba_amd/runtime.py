@@ -90,6 +90,24 @@ class Instance:
         abi.check(self.ctx.lib.dba_kh_vy_mode(self.ctx.h, self.id, C.byref(out)))
         return out.value
 
+    def droplets(self, field, cap=1 << 20):
+        """Droplet analysis on the device: (avg, volume, sizes in the reference's
+        order, mean radius). The radius follows coarsening_analysis: cbrt(3 V /
+        (4 pi N)) in 3D, sqrt(V / (pi N)) in 2D (nan without droplets)."""
+        avg, vol, cnt = C.c_double(), C.c_int64(), C.c_int64()
+        sizes = np.zeros(max(cap, 1), dtype=np.int64)
+        abi.check(self.ctx.lib.dba_droplets(self.ctx.h, self.id, field, C.byref(avg), C.byref(vol), C.byref(cnt),
+                                            sizes.ctypes.data, cap))
+        n = cnt.value
+        sizes = sizes[:min(n, cap)]
+        if n == 0:
+            r = float("nan")
+        elif self.params.nz > 1:
+            r = float(np.cbrt(vol.value * 0.75 / np.pi / n))
+        else:
+            r = (vol.value / np.pi / n) ** 0.5
+        return avg.value, vol.value, sizes, r
+
     def destroy(self):
         if self.id is not None:
             abi.check(self.ctx.lib.dba_instance_destroy(self.ctx.h, self.id))

@@ -121,3 +121,22 @@ def test_missing_file_and_field(tmp_path):
     write_vtk(tmp_path / "b.vtk", [("C", np.zeros((1, 4, 4)))], (1, 4, 4))
     r = subprocess.run([tool(), "-f", "Z", "b.vtk"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "no SCALARS Z" in r.stdout
+
+
+@pytest.mark.parametrize("shape", [(1, 48, 40), (12, 10, 14)])
+def test_droplet_order_matches_device_reference(tmp_path, shape):
+    """The host reference of the device analysis (tests/droplets_ref.py) lists
+    droplets in the order this tool's reference scan finds them, with the
+    reference's threshold and volume."""
+    from droplets_ref import host_droplets
+    rng = np.random.default_rng(5)
+    a = ndimage.uniform_filter(rng.random(shape) ** 4, size=3, mode="wrap")   # sparse droplets
+    a[rng.random(shape) < 0.02] = threshold(a)   # cells equal to the threshold join droplets
+    write_vtk(tmp_path / "c.vtk", [("Z", a)], shape)
+    tool()
+    r = subprocess.run([TOOL, "c.vtk"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout
+    order = [int(x) for x in (tmp_path / "size_dist.txt").read_text().split()]
+    avg, vol, sizes = host_droplets(a)
+    assert order == sizes and len(sizes) > 3
+    assert f"volume = {vol}," in r.stdout

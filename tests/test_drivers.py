@@ -173,7 +173,8 @@ def test_kh_driver_reference_model(tmp_path, mode):
     from oracle.oracle import OracleSim
     n, dt, tt, tf, eps = 128, 2e-4, 60, 20, 8.0
     dx = 1.0 / n
-    r = run("KH_DBA", tmp_path, KH_INPUT.format(n=n, tt=tt, tf=tf, dt=dt, dx=repr(dx), mode=mode, eps=eps, extra=""))
+    r = run("KH_DBA", tmp_path, KH_INPUT.format(n=n, tt=tt, tf=tf, dt=dt, dx=repr(dx), mode=mode, eps=eps,
+                                                 extra="dump 1\n"))
     assert r.returncode == 0, r.stdout + r.stderr
     p = configs.kh_reference_params(n=n, tile=8, mode=mode, dx=dx, dt=dt, eps=eps)
     U0 = configs.kh_reference_ic(n, dx=dx)
@@ -189,6 +190,12 @@ def test_kh_driver_reference_model(tmp_path, mode):
     for t in range(tf, tt + 1, tf):
         o.step(tf)
         U = [o.get_field(f) for f in range(4)]
+        for k, name in enumerate(("Mass", "Momx", "Momy", "Energy")):
+            d = raw(tmp_path / f"{name}_{t}.bin")
+            if mode == abi.MODE_CONSERVED:   # lazy vs eager mean shift: summation order
+                assert np.max(np.abs(d - U[k].ravel())) <= 1e-12 * max(1.0, np.abs(U[k]).max())
+            else:
+                assert np.array_equal(d, U[k].ravel())
         rho, vx, vy, pr = configs.kh_primitive(U, p.gas_gamma, dx * dx)
         v = read_vtk_scalars(tmp_path / f"output_{mode}_{t}.vtk")
         close(v["rho"], rho)
@@ -248,7 +255,8 @@ def test_kh_driver_walls_cfl(tmp_path):
 def test_modelb_driver(tmp_path, mode):
     from oracle.oracle import OracleSim
     n, tt, tf, eps = 16, 40, 20, 1e-3
-    r = run("CH", tmp_path, MB_INPUT.format(nx=n, ny=n, nz=n, tt=tt, tf=tf, dt=0.0005, eps=eps, mode=mode, extra=""))
+    r = run("CH", tmp_path, MB_INPUT.format(nx=n, ny=n, nz=n, tt=tt, tf=tf, dt=0.0005, eps=eps, mode=mode,
+                                            extra="dump 1\ndroplets 1\n"))
     assert r.returncode == 0, r.stdout + r.stderr
     p = configs.modelb_params(n=n, mode=mode, eps=eps)
     X, Y, Z = configs.modelb_ic(n, n, n)
@@ -265,6 +273,8 @@ def test_modelb_driver(tmp_path, mode):
         v = read_vtk_scalars(tmp_path / f"output_{t}.vtk")
         for k, a in (("X", x), ("Y", y), ("Z", z), ("B", 1.0 - x - y - z), ("PHI", phi)):
             close(v[k], a, atol=1e-9)
+        for k, a in (("X", x), ("Y", y), ("Z", z), ("PHI", phi)):   # raw dump: the GPU parity bar
+            assert np.max(np.abs(raw(tmp_path / f"{k}_{t}.bin") - a.ravel())) <= 1e-10
         if mode != abi.MODE_REGULAR:
             wk = read_vtk_scalars(tmp_path / f"Wakeup_{t}.vtk")["wakeup"]
             assert np.array_equal(wk, expand(o.awake(0), (4, 4, 4), (n, n, n)))
@@ -286,14 +296,34 @@ def test_modelb_driver(tmp_path, mode):
     assert list(wp[:, 0]) == list(range(tf, tt + 1, tf))
     expect = 1.0 if mode == abi.MODE_REGULAR else o.awake(0).sum() / o.ntiles
     assert wp[-1, 1] == pytest.approx(expect, abs=1e-6)
+    check_droplets(tmp_path, "Z", [0] + list(range(tf, tt + 1, tf)), tf, (n, n, n))
     o.close()
+
+
+def raw(path):
+    return np.fromfile(path, dtype="<f8")
+
+
+def check_droplets(folder, name, steps, tf, shape):
+    """avg_radius.txt / size_dist.txt of the driver's device analysis against the
+    host reference on the dumped field of each output step."""
+    from droplets_ref import host_droplets, radius
+    rad = [line.split(",") for line in (folder / "avg_radius.txt").read_text().splitlines()]
+    dist = (folder / "size_dist.txt").read_text().splitlines()
+    assert len(rad) == len(steps) == len(dist)
+    for (k, r), sizes, t in zip(rad, dist, steps):
+        _, vol, want = host_droplets(raw(folder / f"{name}_{t}.bin").reshape(shape))
+        assert int(k) == t // tf
+        assert [int(v) for v in sizes.split()] == want
+        if want:
+            assert float(r) == pytest.approx(radius(vol, len(want), shape[0] > 1), rel=1e-5)
 
 
 @pytest.mark.gpu
 def test_ch_driver(tmp_path):
     from oracle.oracle import OracleSim
     n, tt, tf, eps = 128, 60, 30, 0.05
-    extra = "model ch\ntile 32\nmobility 1\nkappa 1\nreact_gamma 1e-3\ncbar 0\namp 0.05\nstencil 9\n"
+    extra = "model ch\ntile 32\nmobility 1\nkappa 1\nreact_gamma 1e-3\ncbar 0\namp 0.05\nstencil 9\ndump 1\ndroplets 1\n"
     r = run("CH", tmp_path, MB_INPUT.format(nx=n, ny=n, nz=1, tt=tt, tf=tf, dt=0.01, eps=eps, mode=1, extra=extra))
     assert r.returncode == 0, r.stdout + r.stderr
     p = configs.ch_params(n=n, tile=32, mode=abi.MODE_DBA, eps=eps)
@@ -302,8 +332,10 @@ def test_ch_driver(tmp_path):
     o.step(tt)
     v = read_vtk_scalars(tmp_path / f"output_{tt}.vtk")
     close(v["C"], o.get_field(0))
+    assert np.array_equal(raw(tmp_path / f"C_{tt}.bin"), o.get_field(0).ravel())   # CH is bit-exact
     wk = read_vtk_scalars(tmp_path / f"Wakeup_{tt}.vtk")["wakeup"]
     assert np.array_equal(wk, expand(o.awake(0), (32, 32, 1), (1, n, n)))
+    check_droplets(tmp_path, "C", [0, tf, tt], tf, (1, n, n))
     o.close()
 
 
@@ -324,13 +356,13 @@ def test_dendrite_driver(tmp_path, model):
     if model == "k94":
         n, nz, tt, tf, dt, dx = 32, 32, 100, 50, 5e-5, 0.02
         text = DEND_INPUT.format(n=n, nz=nz, tt=tt, tf=tf, dt=dt, dx=dx, delta=0.5, tau=3e-4, gamma=40., mode=1,
-                                 extra="vtk 1\n")
+                                 extra="vtk 1\ndump 1\n")
         p = configs.dendrite_k94_params(n=n, nz=nz)
         phi0, T0 = configs.dendrite_k94_ic(n, n, nz)
     else:
         n, nz, tt, tf, dt, dx = 128, 1, 200, 100, 1e-4, 0.03
         text = DEND_INPUT.format(n=n, nz=nz, tt=tt, tf=tf, dt=dt, dx=dx, delta=0.02, tau=3e-4, gamma=10., mode=1,
-                                 extra="model k93\ntile 16\nK 1.6\nj 4\ntheta0 0\nT_eq 1\nbc noflux\nvtk 1\n")
+                                 extra="model k93\ntile 16\nK 1.6\nj 4\ntheta0 0\nT_eq 1\nbc noflux\nvtk 1\ndump 1\n")
         p = configs.dendrite_k93_params(n=n, tile=16)
         phi0, T0 = configs.dendrite_k93_ic(n, n)
         phi0, T0 = phi0[None], T0[None]
@@ -346,6 +378,8 @@ def test_dendrite_driver(tmp_path, model):
         phi = o.get_field(0).reshape(nz, n, n)
         T = o.get_field(1).reshape(nz, n, n)
         close(read_vtk_scalars(tmp_path / f"p_{t}.vtk")["p"], phi, atol=1e-9)
+        assert np.max(np.abs(raw(tmp_path / f"p_{t}.bin") - phi.ravel())) <= 1e-10
+        assert np.max(np.abs(raw(tmp_path / f"T_{t}.bin") - T.ravel())) <= 1e-10
         close(read_vtk_scalars(tmp_path / f"T_{t}.vtk")["T"], T, atol=1e-9)
         d = growth_front(phi, n // 2)
         rows.append([t, (d - prev) / tf, o.awake(0).sum() / o.ntiles, o.awake(1).sum() / o.ntiles])
