@@ -221,6 +221,7 @@ struct Layout {
     int ntx, nty, ntz, nt, nwords;
     int64_t n;
     int nf, nsets, naux;
+    int64_t aux_len;  // doubles per aux array
 };
 
 Layout layout_of(const dba_params& p) {
@@ -235,10 +236,14 @@ Layout layout_of(const dba_params& p) {
     L.nf = nfields_of(p.model);
     L.nsets = nsets_of(p.model);
     L.naux = 0;
+    L.aux_len = L.n;
     if (p.model == DBA_MODEL_MODELB) L.naux = 3;  // chemical potentials (every cell, every mode)
     if (p.mode != DBA_MODE_REGULAR) {
         if (p.model == DBA_MODEL_CH) L.naux = 1;   // mu ring
-        if (p.model == DBA_MODEL_KH) L.naux = 24;  // edge state of sleeping tiles
+        if (p.model == DBA_MODEL_KH) {             // edge state of sleeping tiles: one record per tile
+            L.naux = 1;
+            L.aux_len = (int64_t)L.nt * (p.tile_x == 8 ? KhEdge<8>::R : KhEdge<16>::R);
+        }
     }
     return L;
 }
@@ -247,7 +252,7 @@ int64_t footprint_of(const dba_params& p) {
     Layout L = layout_of(p);
     int64_t b = 0;
     b += 2 * (int64_t)L.nf * L.n * 8;        // double-buffered fields
-    b += (int64_t)L.naux * L.n * 8;          // stored neighbour state
+    b += (int64_t)L.naux * L.aux_len * 8;    // stored neighbour state
     b += 4 * (int64_t)L.nwords * 8 + 2 * (int64_t)L.nt;  // bitmaps + detector bytes
     b += (int64_t)kMaxFields * L.nt * 8;     // tpart
     b += 2 * (int64_t)L.nt * 8;              // tspeed
@@ -766,7 +771,7 @@ dba_status dba_instance_create(dba_ctx* ctx, const dba_params* pp, int32_t* id) 
         ALLOC(h.buf[0][f], (size_t)L.n);
         ALLOC(h.buf[1][f], (size_t)L.n);
     }
-    for (int a = 0; a < L.naux; a++) ALLOC(h.aux[a], (size_t)L.n);
+    for (int a = 0; a < L.naux; a++) ALLOC(h.aux[a], (size_t)L.aux_len);
     for (int s = 0; s < 2; s++) {
         ALLOC(h.a_cur[s], (size_t)L.nwords);
         ALLOC(h.nflag[s], (size_t)L.nt);

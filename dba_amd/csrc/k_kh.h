@@ -15,7 +15,9 @@
 // x-gradients) on its left column, (primes, y-gradients) on its bottom row, the
 // x-face flux of its right column and the y-face flux of its top row.
 // Everything else is recomputed in LDS from the conserved state of the tile
-// and its neighbours (2-cell halo).
+// and its neighbours (2-cell halo). The stored state is one contiguous record
+// per tile (KhEdge), so a tile's edge writes coalesce instead of landing one
+// 8-byte value per cache line down its columns.
 //
 // LDS (TB = 16): primitives 4 x 20^2, primes / x- / y-gradients 12 x 18^2;
 // the x-face fluxes reuse the primitive space, the y-face fluxes the
@@ -69,8 +71,32 @@ __device__ __forceinline__ void kh_rusanov(const KhC& c, double rL, double rR, d
     *fe = flux_Energy - C * 0.5 * (en_L - en_R);
 }
 
+// Edge record of a tile, FP64, field-major: ring primitives r, u, v, p (ring
+// cell order: bottom row, top row, left column, right column, the columns
+// without their corners); left column primes + x-gradients; bottom row primes +
+// y-gradients; x-face fluxes (m, u, v, e) of the right column; y-face fluxes of
+// the top row.
+template <int TB>
+struct KhEdge {
+    static constexpr int RING = 4 * TB - 4;
+    static constexpr int LEFT = 4 * RING;       // [8][TB]: primes, x-gradients by row
+    static constexpr int BOTTOM = LEFT + 8 * TB;  // [8][TB]: primes, y-gradients by column
+    static constexpr int XF = BOTTOM + 8 * TB;  // [4][TB] by row
+    static constexpr int YF = XF + 4 * TB;      // [4][TB] by column
+    static constexpr int R = YF + 4 * TB;       // doubles per tile: 40 TB - 16
+    // slot of tile-local cell (lx, ly) on the ring, -1 inside
+    __device__ __forceinline__ static int ring(int lx, int ly) {
+        if (ly == 0) return lx;
+        if (ly == TB - 1) return TB + lx;
+        if (lx == 0) return 2 * TB + ly - 1;
+        if (lx == TB - 1) return 3 * TB - 2 + ly - 1;
+        return -1;
+    }
+};
+
 template <int TB, int NT>
 __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
+    using E = KhEdge<TB>;
     constexpr int W2 = TB + 4;
     constexpr int W1 = TB + 2;
     constexpr int A2 = W2 * W2;
@@ -129,10 +155,18 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
                 const double ue = I.buf[pp][3][gi] - nb.shift[3][d];
                 kh_prims(c, um, ux, uy, ue, &r, &u, &v, &p);
             } else {
-                r = I.aux[0][gi];
-                u = I.aux[1][gi];
-                v = I.aux[2][gi];
-                p = I.aux[3][gi];
+                // the ring primitives the sleeping owner stored; its inner cells
+                // are never reached by a stencil of a running cell
+                const int ri = E::ring(mx & (TB - 1), my & (TB - 1));
+                if (ri >= 0) {
+                    const gdouble* e = I.aux[0] + (int64_t)nb.tile[d] * E::R;
+                    r = e[ri];
+                    u = e[E::RING + ri];
+                    v = e[2 * E::RING + ri];
+                    p = e[3 * E::RING + ri];
+                } else {
+                    r = u = v = p = 0.0;
+                }
             }
             sP[0 * A2 + k] = r;
             sP[1 * A2 + k] = u;
@@ -148,12 +182,13 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
             const int q = (lx + 2) + (ly + 2) * W2;
             const double r = sP[q], u = sP[A2 + q], v = sP[2 * A2 + q], p = sP[3 * A2 + q];
             if (cfl) vstale = fmax(vstale, kh_speed(c, r, u, v, p));
-            if (dba && (lx == 0 || lx == TB - 1 || ly == 0 || ly == TB - 1)) {
-                const int64_t gi = (x0 + lx) + (int64_t)(y0 + ly) * nx;
-                I.aux[0][gi] = r;
-                I.aux[1][gi] = u;
-                I.aux[2][gi] = v;
-                I.aux[3][gi] = p;
+            const int ri = E::ring(lx, ly);
+            if (dba && ri >= 0) {
+                gdouble* e = I.aux[0] + (int64_t)t * E::R;
+                e[ri] = r;
+                e[E::RING + ri] = u;
+                e[2 * E::RING + ri] = v;
+                e[3 * E::RING + ri] = p;
             }
         }
 
@@ -168,12 +203,19 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
                 // sleeping neighbour: its stored edge state (right: primes + x-grads
                 // of its left column; top: primes + y-grads of its bottom row)
                 if (lx == TB + 1 || ly == TB + 1) {
-                    const int mx = map_index(x0 + lx - 1, nx, 0), my = map_index(y0 + ly - 1, ny, wall ? 1 : 0);
-                    const int64_t gi = mx + (int64_t)my * nx;
-                    for (int q = 0; q < 4; q++) {
-                        sPR[q * A1 + k] = I.aux[4 + q][gi];
-                        if (lx == TB + 1) sGX[q * A1 + k] = I.aux[8 + q][gi];
-                        else sGY[q * A1 + k] = I.aux[12 + q][gi];
+                    const gdouble* e = I.aux[0] + (int64_t)nb.tile[d] * E::R;
+                    if (lx == TB + 1) {
+                        const int row = map_index(y0 + ly - 1, ny, wall ? 1 : 0) & (TB - 1);
+                        for (int q = 0; q < 4; q++) {
+                            sPR[q * A1 + k] = e[E::LEFT + q * TB + row];
+                            sGX[q * A1 + k] = e[E::LEFT + (4 + q) * TB + row];
+                        }
+                    } else {
+                        const int col = map_index(x0 + lx - 1, nx, 0) & (TB - 1);
+                        for (int q = 0; q < 4; q++) {
+                            sPR[q * A1 + k] = e[E::BOTTOM + q * TB + col];
+                            sGY[q * A1 + k] = e[E::BOTTOM + (4 + q) * TB + col];
+                        }
                     }
                 }
                 continue;
@@ -213,22 +255,21 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
                     : fabs(r_x) + fabs(r_y) + fabs(p_x) + fabs(p_y) + fabs(u_x) + fabs(u_y) + fabs(v_x) + fabs(v_y);
                 flag |= metric > I.thresh_a;
                 if (dba && (lx == 1 || ly == 1)) {
-                    const int64_t gi = (x0 + lx - 1) + (int64_t)(y0 + ly - 1) * nx;
-                    I.aux[4][gi] = rp;
-                    I.aux[5][gi] = up;
-                    I.aux[6][gi] = vp;
-                    I.aux[7][gi] = pp;
+                    gdouble* e = I.aux[0] + (int64_t)t * E::R;
+                    const double pr[4] = {rp, up, vp, pp};
                     if (lx == 1) {
-                        I.aux[8][gi] = r_x;
-                        I.aux[9][gi] = u_x;
-                        I.aux[10][gi] = v_x;
-                        I.aux[11][gi] = p_x;
+                        const double gx[4] = {r_x, u_x, v_x, p_x};
+                        for (int q = 0; q < 4; q++) {
+                            e[E::LEFT + q * TB + ly - 1] = pr[q];
+                            e[E::LEFT + (4 + q) * TB + ly - 1] = gx[q];
+                        }
                     }
                     if (ly == 1) {
-                        I.aux[12][gi] = r_y;
-                        I.aux[13][gi] = u_y;
-                        I.aux[14][gi] = v_y;
-                        I.aux[15][gi] = p_y;
+                        const double gy[4] = {r_y, u_y, v_y, p_y};
+                        for (int q = 0; q < 4; q++) {
+                            e[E::BOTTOM + q * TB + lx - 1] = pr[q];
+                            e[E::BOTTOM + (4 + q) * TB + lx - 1] = gy[q];
+                        }
                     }
                 }
             }
@@ -243,12 +284,12 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
             double fm, fe, fu, fv;
             const int d = nb_dir1<TB, 1>(lx) + 3 * nb_dir1<TB, 1>(ly);
             if (lx == 0 && !nb.act[0][d]) {
-                const int mx = map_index(x0 - 1, nx, 0);
-                const int64_t gi = mx + (int64_t)(y0 + ly - 1) * nx;
-                fm = I.aux[16][gi];
-                fu = I.aux[17][gi];
-                fv = I.aux[18][gi];
-                fe = I.aux[19][gi];
+                // the x-face fluxes the sleeping left neighbour kept for its right column
+                const gdouble* e = I.aux[0] + (int64_t)nb.tile[d] * E::R + E::XF + (ly - 1);
+                fm = e[0];
+                fu = e[TB];
+                fv = e[2 * TB];
+                fe = e[3 * TB];
             } else {
                 double L[4], R[4];
                 for (int q = 0; q < 4; q++) {
@@ -257,11 +298,11 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
                 }
                 kh_rusanov(c, L[0], R[0], L[3], R[3], L[1], R[1], L[2], R[2], &fm, &fe, &fu, &fv);
                 if (dba && lx == TB) {
-                    const int64_t gi = (x0 + TB - 1) + (int64_t)(y0 + ly - 1) * nx;
-                    I.aux[16][gi] = fm;
-                    I.aux[17][gi] = fu;
-                    I.aux[18][gi] = fv;
-                    I.aux[19][gi] = fe;
+                    gdouble* e = I.aux[0] + (int64_t)t * E::R + E::XF + (ly - 1);
+                    e[0] = fm;
+                    e[TB] = fu;
+                    e[2 * TB] = fv;
+                    e[3 * TB] = fe;
                 }
             }
             sP[0 * A2 + k] = fm;
@@ -278,12 +319,12 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
             double fm, fe, fu, fv;
             const int d = nb_dir1<TB, 1>(lx) + 3 * nb_dir1<TB, 1>(ly);
             if (ly == 0 && !nb.act[0][d]) {
-                const int my = map_index(y0 - 1, ny, wall ? 1 : 0);
-                const int64_t gi = (x0 + lx - 1) + (int64_t)my * nx;
-                fm = I.aux[20][gi];
-                fu = I.aux[21][gi];
-                fv = I.aux[22][gi];
-                fe = I.aux[23][gi];
+                // the y-face fluxes the sleeping lower neighbour kept for its top row
+                const gdouble* e = I.aux[0] + (int64_t)nb.tile[d] * E::R + E::YF + (lx - 1);
+                fm = e[0];
+                fu = e[TB];
+                fv = e[2 * TB];
+                fe = e[3 * TB];
             } else {
                 double L[4], R[4];
                 for (int q = 0; q < 4; q++) {
@@ -293,11 +334,11 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
                 // momentum roles swapped for y faces (example_Kelvin_Helmholtz/KH_DBA.cu:738-740)
                 kh_rusanov(c, L[0], R[0], L[3], R[3], L[2], R[2], L[1], R[1], &fm, &fe, &fv, &fu);
                 if (dba && ly == TB) {
-                    const int64_t gi = (x0 + lx - 1) + (int64_t)(y0 + TB - 1) * nx;
-                    I.aux[20][gi] = fm;
-                    I.aux[21][gi] = fu;
-                    I.aux[22][gi] = fv;
-                    I.aux[23][gi] = fe;
+                    gdouble* e = I.aux[0] + (int64_t)t * E::R + E::YF + (lx - 1);
+                    e[0] = fm;
+                    e[TB] = fu;
+                    e[2 * TB] = fv;
+                    e[3 * TB] = fe;
                 }
             }
             sGX[0 * A1 + k] = fm;
