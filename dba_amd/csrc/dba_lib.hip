@@ -96,7 +96,6 @@ struct Group {
     SegPart* d_parts = nullptr;
     int32_t* d_counts = nullptr;
     int32_t* d_offsets = nullptr;
-    uint32_t* d_done = nullptr;
     int nseg = 0;
     int64_t total_tiles = 0;
     int grid = 0;
@@ -290,7 +289,6 @@ void free_group(Group* g) {
     (void)hipFree(g->d_parts);
     (void)hipFree(g->d_counts);
     (void)hipFree(g->d_offsets);
-    (void)hipFree(g->d_done);
     delete g;
 }
 
@@ -433,8 +431,6 @@ dba_status get_group(dba_ctx* ctx, const std::vector<int>& ids, Group** out) {
     HIP_TRY(hipMemsetAsync(g->d_parts, 0, sizeof(SegPart) * ns, ctx->setup));
     HIP_TRY(hipMalloc(&g->d_counts, sizeof(int32_t) * ns));
     HIP_TRY(hipMalloc(&g->d_offsets, sizeof(int32_t) * (ns + 1)));
-    HIP_TRY(hipMalloc(&g->d_done, sizeof(uint32_t)));
-    HIP_TRY(hipMemsetAsync(g->d_done, 0, sizeof(uint32_t), ctx->setup));
     HIP_TRY(hipMemsetAsync(g->d_offsets, 0, sizeof(int32_t) * (ns + 1), ctx->setup));
     // the compute stream is used only after this host-side wait
     HIP_TRY(hipStreamSynchronize(ctx->setup));
@@ -511,8 +507,10 @@ void launch_step(const Group* g, hipStream_t s) {
 
 void launch_finalize(const Group* g, hipStream_t s, int rotate) {
     hipLaunchKernelGGL(finalize_kernel, dim3((unsigned)g->nseg), dim3(kFinNT), 0, s, g->d_insts, g->d_segs,
-                       g->nseg, g->d_isegs, (int)g->ids.size(), g->d_parts, g->d_counts, g->d_offsets, g->d_done,
-                       rotate);
+                       g->nseg, g->d_parts, g->d_counts, g->d_offsets, rotate);
+    if (g->nseg > 1)
+        hipLaunchKernelGGL(finalize_merge_kernel, dim3(1), dim3(kFinNT), 0, s, g->d_insts, g->nseg, g->d_isegs,
+                           (int)g->ids.size(), g->d_parts, g->d_counts, g->d_offsets, rotate);
 }
 
 hipEvent_t pooled_event(dba_ctx* ctx) {

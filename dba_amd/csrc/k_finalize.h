@@ -158,7 +158,6 @@ struct FinLds {
     double s_scal[16];
     int64_t s_step;
     int64_t s_upd;
-    int s_last;
 };
 
 // One thread per tile of the segment, one wave per 64-tile bitmap word, so the
@@ -166,9 +165,8 @@ struct FinLds {
 // the conserved and CFL variants).
 template <int NT>
 __device__ __forceinline__ void finalize_body(const Inst* __restrict__ insts, const Seg* __restrict__ segs,
-                                              int segi, int nseg, const InstSegs* __restrict__ isegs, int n_inst,
-                                              SegPart* __restrict__ parts, int32_t* __restrict__ counts,
-                                              int32_t* __restrict__ offsets, uint32_t* __restrict__ done,
+                                              int segi, int nseg, SegPart* __restrict__ parts,
+                                              int32_t* __restrict__ counts, int32_t* __restrict__ offsets,
                                               int rotate, FinLds& F) {
     static_assert(NT == kSegTiles, "one thread per tile of a segment");
     auto& s_pb = F.s_pb;
@@ -181,7 +179,6 @@ __device__ __forceinline__ void finalize_body(const Inst* __restrict__ insts, co
     auto& s_scal = F.s_scal;
     auto& s_step = F.s_step;
     auto& s_upd = F.s_upd;
-    auto& s_last = F.s_last;
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     // one segment: instance 0, all of its tiles (no dependent load of the segment table)
@@ -338,16 +335,23 @@ __device__ __forceinline__ void finalize_body(const Inst* __restrict__ insts, co
             for (int f = 0; f < kMaxFields; f++) parts[segi].csum[f] = f < I.ncons ? s_cs[f] : 0.0;
             parts[segi].vmax = vmax;
         }
-        __threadfence();
-        const unsigned prev = atomicAdd(done, 1u);
-        s_last = (prev == (unsigned)(nseg - 1));
     }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    // one wave per instance: lanes stride over its segments, then a fixed-shape
-    // shuffle tree (the same order on every run, and for a one-segment instance
-    // exactly that segment's partial)
+}
+
+// Several segments: one workgroup merges the segment partials after the
+// segment workgroups (the kernel boundary makes their writes visible; a
+// last-workgroup-done counter would need an agent-scope fence in every
+// segment workgroup, and each such fence writes back and invalidates its XCD's
+// L2). Per instance, one wave: lanes stride over the instance's segments, then a
+// fixed-shape shuffle tree (the same order on every run, and for a one-segment
+// instance exactly that segment's partial). Then the merged list offsets.
+template <int NT>
+__device__ __forceinline__ void finalize_merge_body(const Inst* __restrict__ insts, int nseg,
+                                                    const InstSegs* __restrict__ isegs, int n_inst,
+                                                    const SegPart* __restrict__ parts,
+                                                    const int32_t* __restrict__ counts,
+                                                    int32_t* __restrict__ offsets, int rotate, int* si) {
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     for (int i0 = 0; i0 < n_inst; i0 += NT / 64) {
         const int i = i0 + wid;
         if (i >= n_inst) break;
@@ -392,22 +396,26 @@ __device__ __forceinline__ void finalize_body(const Inst* __restrict__ insts, co
         if (i < nseg) offsets[i] = running + ex;
         running += tot;
     }
-    if (tid == 0) {
-        offsets[nseg] = running;
-        *done = 0u;
-    }
+    if (tid == 0) offsets[nseg] = running;
 }
 
 
 __global__ __launch_bounds__(kFinNT) void finalize_kernel(const Inst* __restrict__ insts,
                                                           const Seg* __restrict__ segs, int nseg,
-                                                          const InstSegs* __restrict__ isegs, int n_inst,
                                                           SegPart* __restrict__ parts,
                                                           int32_t* __restrict__ counts,
-                                                          int32_t* __restrict__ offsets,
-                                                          uint32_t* __restrict__ done, int rotate) {
+                                                          int32_t* __restrict__ offsets, int rotate) {
     __shared__ FinLds F;
-    finalize_body<kFinNT>(insts, segs, blockIdx.x, nseg, isegs, n_inst, parts, counts, offsets, done, rotate, F);
+    finalize_body<kFinNT>(insts, segs, blockIdx.x, nseg, parts, counts, offsets, rotate, F);
+}
+
+__global__ __launch_bounds__(kFinNT) void finalize_merge_kernel(const Inst* __restrict__ insts, int nseg,
+                                                                const InstSegs* __restrict__ isegs, int n_inst,
+                                                                const SegPart* __restrict__ parts,
+                                                                const int32_t* __restrict__ counts,
+                                                                int32_t* __restrict__ offsets, int rotate) {
+    __shared__ int si[kFinNT / 64 + 2];
+    finalize_merge_body<kFinNT>(insts, nseg, isegs, n_inst, parts, counts, offsets, rotate, si);
 }
 
 // Current value of one field: the buffer the tile parity selects, minus the
