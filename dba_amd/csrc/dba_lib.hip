@@ -238,7 +238,10 @@ Layout layout_of(const dba_params& p) {
     L.aux_len = L.n;
     if (p.model == DBA_MODEL_MODELB) L.naux = 3;  // chemical potentials (every cell, every mode)
     if (p.mode != DBA_MODE_REGULAR) {
-        if (p.model == DBA_MODEL_CH) L.naux = 1;   // mu ring
+        if (p.model == DBA_MODEL_CH) {             // mu ring: one record per tile
+            L.naux = 1;
+            L.aux_len = (int64_t)L.nt * (4 * p.tile_x - 4);
+        }
         if (p.model == DBA_MODEL_KH) {             // edge state of sleeping tiles: one record per tile
             L.naux = 1;
             L.aux_len = (int64_t)L.nt * (p.tile_x == 8 ? KhEdge<8>::R : KhEdge<16>::R);

@@ -18,11 +18,27 @@
 //   3. c' = c + dt (M lap mu - Gamma (c - cbar)) into the other buffer;
 //   4. detector any(|lap c| > eps) -> one byte per tile; the tile's own
 //      edge-ring mu is stored for its neighbours.
-// HBM traffic per cell: read c, write c' (16 B) + the mu ring (4(T-1)/T^2 of cells).
+// HBM traffic per cell: read c, write c' (16 B) + the mu ring (4(T-1)/T^2 of cells),
+// kept as one contiguous record of 4 TB - 4 values per tile (ring order of
+// TileRing) so that its column cells do not each cost a partial-line write.
 #pragma once
 #include "common.h"
 
 namespace dba {
+
+// Ring cells of a TB x TB tile, in record order: bottom row, top row, left
+// column, right column (the columns without their corners).
+template <int TB>
+struct TileRing {
+    static constexpr int N = 4 * TB - 4;
+    __device__ __forceinline__ static int slot(int lx, int ly) {
+        if (ly == 0) return lx;
+        if (ly == TB - 1) return TB + lx;
+        if (lx == 0) return 2 * TB + ly - 1;
+        if (lx == TB - 1) return 3 * TB - 2 + ly - 1;
+        return -1;
+    }
+};
 
 // Column-strip register blocking: thread (tx, ty) owns rows ty*R .. ty*R+R-1
 // of column tx, R = TB*TB/NT. A 9-point stencil over its R cells reads
@@ -106,8 +122,8 @@ __device__ __forceinline__ void ch_tile(CInst& I, int t, uint64_t rec, NbTable& 
             flag |= fabs(lc[r]) > eps;
             const int ly = ty * R + r;
             smu[(tx + 1) + (ly + 1) * W1] = mu;
-            if (!regular && (tx == 0 || tx == TB - 1 || ly == 0 || ly == TB - 1))
-                mu_store[(x0 + tx) + (int64_t)(y0 + ly) * nx] = mu;
+            const int slot = TileRing<TB>::slot(tx, ly);
+            if (!regular && slot >= 0) mu_store[(int64_t)t * TileRing<TB>::N + slot] = mu;
         }
     }
     for (int k = tid; k < NH1; k += NT) {
@@ -122,8 +138,11 @@ __device__ __forceinline__ void ch_tile(CInst& I, int t, uint64_t rec, NbTable& 
             const double cv = sc[(lx + 1) + (ly + 1) * W2];
             mu = (cv * cv * cv - cv) - kappa * lcv;
         } else {
+            // the ring value the sleeping owner stored (a ring cell of it: the map
+            // only lands inside the running tile itself)
             const int mx = map_index(x0 + lx - 1, nx, bcx), my = map_index(y0 + ly - 1, ny, bcy);
-            mu = mu_store[mx + (int64_t)my * nx];
+            const int sl = TileRing<TB>::slot(mx & (TB - 1), my & (TB - 1));
+            mu = sl >= 0 ? mu_store[(int64_t)nb.tile[d] * TileRing<TB>::N + sl] : 0.0;
         }
         smu[lx + ly * W1] = mu;
     }
