@@ -19,8 +19,12 @@ A request's result does not depend on the others: instances share launches but
 no data, so its fields are bit-identical alone and in the mix.
 
 The arrival rate can be steered during the run toward a target memory
-occupancy (an inhomogeneous Poisson process whose rate is multiplied by
-exp(gain (target - occupancy)) after every tick, clamped to [rate/8, 8 rate]).
+demand (an inhomogeneous Poisson process whose rate is multiplied after every
+tick by exp(gain (target - demand) dt), by at most e^0.25 either way per tick and within
+[rate/8, 8 rate]). Demand counts the footprint of the queued requests as well as
+the resident ones: steering on occupancy alone keeps lowering the rate while a
+backlog drains and raises it only after the memory has emptied, so the rate
+swings between its clamps and the device idles between swings.
 """
 import math
 import time
@@ -195,7 +199,10 @@ class Server:
                         break
                     todo[0].arrival = next_t
                     next_t += -math.log1p(-next(u_iter)) / rate
-                queue.append(todo.popleft())
+                r = todo.popleft()
+                if r.rid not in fps:
+                    fps[r.rid] = self.b.footprint(r.params(self.tiles, self.threads))
+                queue.append(r)
 
         def admit():
             """FIFO admission while the head of the queue fits the budget."""
@@ -248,8 +255,9 @@ class Server:
                 for v in resident.values():
                     v[1] -= k
                 if rate is not None and self.target is not None:
-                    err = self.target - self.used / self.capacity
-                    rate = min(8 * rate0, max(rate0 / 8, rate * math.exp(self.gain * err * (now - last_tick))))
+                    demand = (self.used + sum(fps[q.rid] for q in queue)) / self.capacity
+                    f = math.exp(min(0.25, max(-0.25, self.gain * (self.target - demand) * (now - last_tick))))
+                    rate = min(8 * rate0, max(rate0 / 8, rate * f))
                     self.rate_trace.append((now, rate))
                 last_tick = now
             resident.update(incoming)
