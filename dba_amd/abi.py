@@ -103,6 +103,8 @@ class Info(C.Structure):
         ("cons_offset", C.c_double),
         ("footprint_bytes", C.c_int64),
         ("cell_updates", C.c_int64),
+        ("single_launch", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -133,6 +135,7 @@ SIGNATURES = [
     ("dba_field_sum", C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
     ("dba_growth_front", C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p]),
     ("dba_kh_vy_mode", C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+    ("dba_set_single_launch", C.c_int32, [C.c_void_p, C.c_int32]),
     ("dba_droplets", C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                  C.POINTER(C.c_int64), C.c_void_p, C.c_int64]),
 ]

@@ -34,6 +34,9 @@ struct Inst {
     gdouble* aux[kMaxAux];        // model state kept for sleeping-neighbour semantics
     gu64* a_cur[2];               // awake bitmap of the step being executed, per set
     gu8* nflag[2];                // detector output per tile (Wakeup_Next), per set
+    gu8* dF[2];                   // single-launch steps: detector bit per tile, by step parity
+    gu8* dP[2];                   // ... buffer parity per tile, by step parity
+    gi32* druns;                  // ... runs per tile since the last canonicalisation
     gu64* par[2];                 // per-tile buffer parity per set: bit=1 -> buf[1] is current
     gdouble* tpart;               // conserved: per-tile change sums, [field][nt]
     gdouble* tspeed;              // KH: per-tile max wave speed, [0..nt) fresh (current U), [nt..2nt) stale

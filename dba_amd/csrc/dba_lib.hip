@@ -133,6 +133,7 @@ struct dba_ctx {
     double* d_diag = nullptr;      // diagnostics: block partials, results
     int32_t* d_cols = nullptr;     // diagnostics: growth-front columns
     bool timing = false;
+    bool single_launch = true;     // single-launch steps for eligible groups (dba_set_single_launch)
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_step, ev_fin;
     std::vector<hipEvent_t> ev_pool;
     hipEvent_t markers[8] = {};
@@ -508,6 +509,44 @@ void launch_step(const Group* g, hipStream_t s) {
     }
 }
 
+// Single-launch steps: one instance in one segment (its whole state fits one
+// canonicalising workgroup), a model with a single-launch kernel, no conserved
+// shift (its per-step global sum needs the finalize).
+bool single_launch_params(const dba_params& p, int nt) {
+    return nt <= kSegTiles && p.model == DBA_MODEL_CH && p.mode != DBA_MODE_CONSERVED;
+}
+
+bool single_launch_ok(const dba_ctx* ctx, const Group* g) {
+    if (!ctx->single_launch || g->ids.size() != 1 || g->nseg != 1) return false;
+    return single_launch_params(ctx->inst.at(g->ids[0])->p, (int)g->total_tiles);
+}
+
+template <int TB, int NT>
+void launch_dense_tb(const Group* g, hipStream_t s, int first, int step) {
+    constexpr int NTC = NT <= TB * TB ? NT : TB * TB;
+    hipLaunchKernelGGL((ch_dense_kernel<TB, NTC>), dim3((unsigned)g->total_tiles), dim3(NTC), 0, s, g->d_insts,
+                       first, step);
+}
+
+template <int TB>
+void launch_dense_nt(const Group* g, hipStream_t s, int first, int step) {
+    switch (g->threads) {
+        case 64: launch_dense_tb<TB, 64>(g, s, first, step); break;
+        case 128: launch_dense_tb<TB, 128>(g, s, first, step); break;
+        case 512: launch_dense_tb<TB, 512>(g, s, first, step); break;
+        case 1024: launch_dense_tb<TB, 1024>(g, s, first, step); break;
+        default: launch_dense_tb<TB, 256>(g, s, first, step); break;
+    }
+}
+
+void launch_dense(const Group* g, hipStream_t s, int first, int step) {
+    switch (g->tile) {
+        case 8: launch_dense_nt<8>(g, s, first, step); break;
+        case 16: launch_dense_nt<16>(g, s, first, step); break;
+        default: launch_dense_nt<32>(g, s, first, step); break;
+    }
+}
+
 void launch_finalize(const Group* g, hipStream_t s, int rotate) {
     hipLaunchKernelGGL(finalize_kernel, dim3((unsigned)g->nseg), dim3(kFinNT), 0, s, g->d_insts, g->d_segs,
                        g->nseg, g->d_parts, g->d_counts, g->d_offsets, rotate);
@@ -556,8 +595,30 @@ dba_status partition(dba_ctx* ctx, const int32_t* ids, int32_t n_ids, std::vecto
 }
 
 dba_status enqueue_steps(dba_ctx* ctx, const std::vector<Group*>& gs, int n_steps, bool timed) {
+    // single-launch groups: n steps, then one canonicalisation
+    for (Group* g : gs) {
+        if (n_steps == 0 || !single_launch_ok(ctx, g)) continue;
+        for (int s = 0; s < n_steps + 1; s++) {
+            hipEvent_t a = nullptr, b = nullptr;
+            if (timed) {
+                a = pooled_event(ctx);
+                b = pooled_event(ctx);
+                (void)hipEventRecord(a, ctx->stream);
+            }
+            if (s < n_steps)
+                launch_dense(g, ctx->stream, s == 0, s);
+            else
+                hipLaunchKernelGGL(dense_canon_kernel, dim3(1), dim3(kFinNT), 0, ctx->stream, g->d_insts, n_steps);
+            LAUNCH_CHECK();
+            if (timed) {
+                (void)hipEventRecord(b, ctx->stream);
+                (s < n_steps ? ctx->ev_step : ctx->ev_fin).push_back({a, b});
+            }
+        }
+    }
     for (int s = 0; s < n_steps; s++) {
         for (Group* g : gs) {
+            if (single_launch_ok(ctx, g)) continue;
             hipEvent_t a = nullptr, b = nullptr, c = nullptr;
             if (timed) {
                 a = pooled_event(ctx);
@@ -777,7 +838,10 @@ dba_status dba_instance_create(dba_ctx* ctx, const dba_params* pp, int32_t* id) 
         ALLOC(h.a_cur[s], (size_t)L.nwords);
         ALLOC(h.nflag[s], (size_t)L.nt);
         ALLOC(h.par[s], (size_t)L.nwords);
+        ALLOC(h.dF[s], (size_t)L.nt);
+        ALLOC(h.dP[s], (size_t)L.nt);
     }
+    ALLOC(h.druns, (size_t)L.nt);
     ALLOC(h.tpart, (size_t)kMaxFields * L.nt);
     ALLOC(h.tspeed, (size_t)2 * L.nt);
     ALLOC(h.s_tile, (size_t)kMaxFields * 2 * L.nt);
@@ -840,6 +904,8 @@ dba_status dba_instance_info(dba_ctx* ctx, int32_t id, dba_info* out) {
     out->last_dt = scal[3];
     out->cons_offset = scal[8];
     out->footprint_bytes = in->footprint;
+    out->single_launch = ctx->single_launch && single_launch_params(in->p, in->h.nt) ? 1 : 0;
+    out->reserved = 0;
     return DBA_OK;
 }
 
@@ -919,6 +985,19 @@ dba_status dba_step_graph(dba_ctx* ctx, const int32_t* ids, int32_t n_ids, int32
         for (int r = 0; r < reps; r++) HIP_TRY(hipGraphLaunch(exec, ctx->stream));
     }
     if (rem > 0) return enqueue_steps(ctx, gs, rem, false);
+    return DBA_OK;
+}
+
+dba_status dba_set_single_launch(dba_ctx* ctx, int32_t enable) {
+    if (!ctx) return fail(DBA_ERR_INVALID, "null ctx");
+    const bool on = enable != 0;
+    if (on != ctx->single_launch) {
+        // captured graphs hold the other launch sequence
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second.exec);
+        ctx->graphs.clear();
+        ctx->single_launch = on;
+    }
     return DBA_OK;
 }
 

@@ -23,6 +23,7 @@
 // TileRing) so that its column cells do not each cost a partial-line write.
 #pragma once
 #include "common.h"
+#include "k_finalize.h"
 
 namespace dba {
 
@@ -65,9 +66,13 @@ __device__ __forceinline__ void strip_lap(const double* s, int cx, int cy0, doub
     }
 }
 
-template <int TB, int NT, bool NINE>
-__device__ __forceinline__ void ch_tile(CInst& I, int t, uint64_t rec, NbTable& nb, double* sc, double* smu,
-                                        double* sred, int* sflag) {
+// One tile's step. WORKLIST: fill the neighbour table from the record and store
+// the detector byte here; otherwise the caller has filled the table (then a
+// barrier) and stores the returned detector bit any(|lap c| > eps) (0 in regular
+// mode, uniform over the workgroup) itself.
+template <int TB, int NT, bool NINE, bool WORKLIST>
+__device__ __forceinline__ int ch_tile(CInst& I, int t, uint64_t rec, NbTable& nb, double* sc, double* smu,
+                                       double* sred, int* sflag) {
     constexpr int W2 = TB + 4;
     constexpr int W1 = TB + 2;
     constexpr int R = TB * TB / NT;
@@ -84,8 +89,10 @@ __device__ __forceinline__ void ch_tile(CInst& I, int t, uint64_t rec, NbTable& 
     gdouble* __restrict__ mu_store = I.aux[0];
     const double kappa = I.kappa, eps = I.thresh_a;
     const double inv = NINE ? I.inv6dx2 : I.invdx2;
-    nb_fill<TB>(nb, I, t, x0, y0, regular, 1, cons ? 1 : 0, rec);
-    __syncthreads();
+    if (WORKLIST) {
+        nb_fill<TB>(nb, I, t, x0, y0, regular, 1, cons ? 1 : 0, rec);
+        __syncthreads();
+    }
     const int pself = nb.par[0][4];
     const double sh_self = nb.shift[0][4];
     const gdouble* __restrict__ cur = pself ? b1 : b0;
@@ -162,11 +169,9 @@ __device__ __forceinline__ void ch_tile(CInst& I, int t, uint64_t rec, NbTable& 
         }
     }
 
-    // 4. detector byte, conserved bookkeeping
-    if (!regular) {
-        const int any = block_any<NT>(flag, sflag);
-        if (tid == 0) I.nflag[0][t] = (uint8_t)any;
-    }
+    // 4. detector bit, conserved bookkeeping
+    const int any = regular ? 0 : block_any<NT>(flag, sflag);
+    if (WORKLIST && !regular && tid == 0) I.nflag[0][t] = (uint8_t)any;
     if (cons) {
         const double tot = block_sum<NT>(part, sred);
         if (tid == 0) {
@@ -175,6 +180,7 @@ __device__ __forceinline__ void ch_tile(CInst& I, int t, uint64_t rec, NbTable& 
         }
     }
     __syncthreads();
+    return any;
 }
 
 template <int TB, int NT>
@@ -197,8 +203,66 @@ __global__ __launch_bounds__(NT) void ch_step_kernel(const WorkList wl) {
         uint64_t rec;
         if (!work_next(wl, cur, total, &Ip, &t, &rec)) break;
         CInst& I = *Ip;
-        if (I.stencil != 5) ch_tile<TB, NT, true>(I, t, rec, nb, sc, smu, sred, sflag);
-        else ch_tile<TB, NT, false>(I, t, rec, nb, sc, smu, sred, sflag);
+        if (I.stencil != 5) ch_tile<TB, NT, true, true>(I, t, rec, nb, sc, smu, sred, sflag);
+        else ch_tile<TB, NT, false, true>(I, t, rec, nb, sc, smu, sred, sflag);
+    }
+}
+
+// Single-launch DBA step of a single-instance CH group: one workgroup per
+// tile and no worklist or finalize launch. Workgroup t derives the awake bit and
+// buffer parity of its 3x3 owner tiles itself: at the first step of a run from
+// the bitmaps, afterwards from the per-tile bytes the previous step wrote
+// (awake = the dilation of the detector bits, exactly the finalize's rule). A
+// sleeping tile writes its bytes and exits; an awake one runs ch_tile as in the
+// worklist kernel. dense_canon_kernel restores the bitmaps and counters at the
+// end of the run. Results are bit-identical to the worklist path.
+template <int TB, int NT>
+__global__ __launch_bounds__(NT) void ch_dense_kernel(const Inst* __restrict__ insts, int first, int s) {
+    constexpr int W2 = TB + 4;
+    constexpr int W1 = TB + 2;
+    __shared__ double sc[W2 * W2];
+    __shared__ double smu[W1 * W1];
+    __shared__ double sred[NT / 64 + 1];
+    __shared__ int sflag[NT / 64 + 1];
+    __shared__ NbTable nb;
+    CInst& I = ((CInst*)insts)[0];
+    // the worklist kernels' XCD deal: contiguous tile ranges per XCD, so halo
+    // rows written by a neighbour tile last step are in this XCD's L2
+    const int t = xcd_deal(blockIdx.x, gridDim.x, 0, gridDim.x);
+    const int prev = s & 1, next = prev ^ 1;
+    const bool regular = I.mode == 0;
+    const int d = threadIdx.x;
+    if (d < 9) {
+        const int ntx = I.ntx, tj = t / ntx, ti = t - tj * ntx;
+        const int o = nb_owner2d(I, ti, tj, d);
+        int par, act;
+        if (first) {
+            par = bit_of(I.par[0], o);
+            act = regular ? 1 : bit_of(I.a_cur[0], o);
+        } else {
+            par = I.dP[prev][o];
+            const int oj = o / ntx, oi = o - oj * ntx;
+            act = regular ? 1 : dilated2d(I.dF[prev], oi, oj, ntx, nb2_of(oi, oj, ntx, I.nty, I.bc_x, I.bc_y), 0);
+        }
+        nb.tile[d] = o;
+        nb.par[0][d] = par;
+        nb.act[0][d] = act;
+        for (int f = 0; f < kMaxFields; f++) nb.shift[f][d] = 0.0;
+    }
+    __syncthreads();
+    const int pself = nb.par[0][4], ran = nb.act[0][4];
+    int any = 0;
+    if (ran)
+        any = I.stencil != 5 ? ch_tile<TB, NT, true, false>(I, t, 0, nb, sc, smu, sred, sflag)
+                             : ch_tile<TB, NT, false, false>(I, t, 0, nb, sc, smu, sred, sflag);
+    if (threadIdx.x == 0) {
+        I.dF[next][t] = (uint8_t)any;
+        I.dP[next][t] = (uint8_t)(pself ^ ran);
+        if (ran) {
+            I.druns[t] += 1;
+            const int64_t step = I.ctr[0] + s;
+            if (step < I.hist_cap) atomicAdd((int*)&I.hist[2 * step], 1);
+        }
     }
 }
 

@@ -418,6 +418,66 @@ __global__ __launch_bounds__(kFinNT) void finalize_merge_kernel(const Inst* __re
     finalize_merge_body<kFinNT>(insts, nseg, isegs, n_inst, parts, counts, offsets, rotate, si);
 }
 
+// End of a run of n >= 1 single-launch steps of a single-instance 2D group (one
+// set, at most one segment): the state the worklist path and the host read,
+// from the per-tile bytes of the last step. Awake bitmap = the dilation of the
+// last detector bits, parity bitmap, detector bytes (a tile that did not fire
+// holds 0, the invariant the finalize's dilation relies on), steps, cell
+// updates, the awake count, and the simulated time advanced step by step as the
+// finalize advances it. One thread per tile.
+__global__ __launch_bounds__(kFinNT) void dense_canon_kernel(const Inst* __restrict__ insts, int n) {
+    __shared__ unsigned long long scells[kFinNT / 64];
+    __shared__ int scnt[kFinNT / 64];
+    CInst& I = ((CInst*)insts)[0];
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    const int nt = I.nt, ntx = I.ntx;
+    const bool valid = t < nt;
+    const bool regular = I.mode == 0;
+    const int last = n & 1;
+    int f = 0, p = 0, a = 0, r = 0;
+    if (valid) {
+        f = I.dF[last][t];
+        p = I.dP[last][t];
+        r = I.druns[t];
+        const int tj = t / ntx, ti = t - tj * ntx;
+        a = regular ? 1 : dilated2d(I.dF[last], ti, tj, ntx, nb2_of(ti, tj, ntx, I.nty, I.bc_x, I.bc_y), 0);
+    }
+    const uint64_t wa = __ballot(a), wp = __ballot(p);
+    const int nw = (nt + 63) >> 6;
+    if (lane == 0 && wid < nw) {
+        I.par[0][wid] = wp;
+        if (!regular) I.a_cur[0][wid] = wa;
+    }
+    if (valid) {
+        I.nflag[0][t] = (uint8_t)f;
+        I.druns[t] = 0;
+    }
+    unsigned long long cells = valid ? (unsigned long long)r * (unsigned long long)(I.tx * I.ty * I.tz) : 0ull;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) cells += __shfl_down(cells, off, 64);
+    if (lane == 0) {
+        scells[wid] = cells;
+        scnt[wid] = __popcll(wa);
+    }
+    __syncthreads();
+    if (t == 0) {
+        unsigned long long c = 0;
+        int cnt = 0;
+        for (int w = 0; w < nw; w++) {
+            c += scells[w];
+            cnt += scnt[w];
+        }
+        I.ctr[0] = I.ctr[0] + n;
+        I.ctr[1] = I.ctr[1] + (int64_t)c;
+        I.ctr[2] = cnt;
+        const double dt = I.scal[1];
+        double tm = I.scal[2];
+        for (int i = 0; i < n; i++) tm += dt;
+        I.scal[2] = tm;
+        I.scal[3] = dt;
+    }
+}
+
 // Current value of one field: the buffer the tile parity selects, minus the
 // conserved shift accumulated since the tile last ran.
 __global__ void materialize_kernel(const Inst* __restrict__ insts, int slot, int field, double* __restrict__ dst) {
