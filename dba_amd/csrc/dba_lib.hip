@@ -513,7 +513,8 @@ void launch_step(const Group* g, hipStream_t s) {
 // canonicalising workgroup), a model with a single-launch kernel, no conserved
 // shift (its per-step global sum needs the finalize).
 bool single_launch_params(const dba_params& p, int nt) {
-    return nt <= kSegTiles && p.model == DBA_MODEL_CH && p.mode != DBA_MODE_CONSERVED;
+    return nt <= kSegTiles && (p.model == DBA_MODEL_CH || p.model == DBA_MODEL_DENDRITE_K93) &&
+           p.mode != DBA_MODE_CONSERVED;
 }
 
 bool single_launch_ok(const dba_ctx* ctx, const Group* g) {
@@ -523,9 +524,13 @@ bool single_launch_ok(const dba_ctx* ctx, const Group* g) {
 
 template <int TB, int NT>
 void launch_dense_tb(const Group* g, hipStream_t s, int first, int step) {
-    constexpr int NTC = NT <= TB * TB ? NT : TB * TB;
-    hipLaunchKernelGGL((ch_dense_kernel<TB, NTC>), dim3((unsigned)g->total_tiles), dim3(NTC), 0, s, g->d_insts,
-                       first, step);
+    const dim3 grid((unsigned)g->total_tiles);
+    if (g->model == DBA_MODEL_CH) {
+        constexpr int NTC = NT <= TB * TB ? NT : TB * TB;
+        hipLaunchKernelGGL((ch_dense_kernel<TB, NTC>), grid, dim3(NTC), 0, s, g->d_insts, first, step);
+    } else {
+        hipLaunchKernelGGL((k93_dense_kernel<TB, NT>), grid, dim3(NT), 0, s, g->d_insts, first, step);
+    }
 }
 
 template <int TB>
@@ -838,8 +843,8 @@ dba_status dba_instance_create(dba_ctx* ctx, const dba_params* pp, int32_t* id) 
         ALLOC(h.a_cur[s], (size_t)L.nwords);
         ALLOC(h.nflag[s], (size_t)L.nt);
         ALLOC(h.par[s], (size_t)L.nwords);
-        ALLOC(h.dF[s], (size_t)L.nt);
-        ALLOC(h.dP[s], (size_t)L.nt);
+        ALLOC(h.dF[s], (size_t)L.nsets * L.nt);
+        ALLOC(h.dP[s], (size_t)L.nsets * L.nt);
     }
     ALLOC(h.druns, (size_t)L.nt);
     ALLOC(h.tpart, (size_t)kMaxFields * L.nt);

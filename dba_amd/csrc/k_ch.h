@@ -231,24 +231,7 @@ __global__ __launch_bounds__(NT) void ch_dense_kernel(const Inst* __restrict__ i
     const int t = xcd_deal(blockIdx.x, gridDim.x, 0, gridDim.x);
     const int prev = s & 1, next = prev ^ 1;
     const bool regular = I.mode == 0;
-    const int d = threadIdx.x;
-    if (d < 9) {
-        const int ntx = I.ntx, tj = t / ntx, ti = t - tj * ntx;
-        const int o = nb_owner2d(I, ti, tj, d);
-        int par, act;
-        if (first) {
-            par = bit_of(I.par[0], o);
-            act = regular ? 1 : bit_of(I.a_cur[0], o);
-        } else {
-            par = I.dP[prev][o];
-            const int oj = o / ntx, oi = o - oj * ntx;
-            act = regular ? 1 : dilated2d(I.dF[prev], oi, oj, ntx, nb2_of(oi, oj, ntx, I.nty, I.bc_x, I.bc_y), 0);
-        }
-        nb.tile[d] = o;
-        nb.par[0][d] = par;
-        nb.act[0][d] = act;
-        for (int f = 0; f < kMaxFields; f++) nb.shift[f][d] = 0.0;
-    }
+    dense_nb_fill(nb, I, t, first, prev, regular, 1);
     __syncthreads();
     const int pself = nb.par[0][4], ran = nb.act[0][4];
     int any = 0;

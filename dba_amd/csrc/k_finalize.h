@@ -418,8 +418,37 @@ __global__ __launch_bounds__(kFinNT) void finalize_merge_kernel(const Inst* __re
     finalize_merge_body<kFinNT>(insts, nseg, isegs, n_inst, parts, counts, offsets, rotate, si);
 }
 
-// End of a run of n >= 1 single-launch steps of a single-instance 2D group (one
-// set, at most one segment): the state the worklist path and the host read,
+// Neighbour table of a single-launch step (threads 0..8; caller synchronises):
+// per set, buffer parity and awake bit of each 3x3 owner tile, from the bitmaps
+// at the first step of a run, afterwards from the previous step's per-tile bytes
+// (awake = the 5-point dilation of its detector bits, the finalize's rule).
+// dF / dP hold, per step parity, nsets x nt bytes (set-major).
+__device__ __forceinline__ void dense_nb_fill(NbTable& nb, CInst& I, int t, int first, int prev, bool regular,
+                                              int nsets) {
+    const int d = threadIdx.x;
+    if (d >= 9) return;
+    const int nt = I.nt, ntx = I.ntx, tj = t / ntx, ti = t - tj * ntx;
+    const int o = nb_owner2d(I, ti, tj, d);
+    const int oj = o / ntx, oi = o - oj * ntx;
+    nb.tile[d] = o;
+    for (int s = 0; s < nsets; s++) {
+        int par, act;
+        if (first) {
+            par = bit_of(I.par[s], o);
+            act = regular ? 1 : bit_of(I.a_cur[s], o);
+        } else {
+            par = I.dP[prev][s * nt + o];
+            act = regular ? 1
+                          : dilated2d(I.dF[prev] + s * nt, oi, oj, ntx, nb2_of(oi, oj, ntx, I.nty, I.bc_x, I.bc_y), 0);
+        }
+        nb.par[s][d] = par;
+        nb.act[s][d] = act;
+    }
+    for (int f = 0; f < kMaxFields; f++) nb.shift[f][d] = 0.0;
+}
+
+// End of a run of n >= 1 single-launch steps of a single-instance 2D group (at
+// most one segment): the state the worklist path and the host read,
 // from the per-tile bytes of the last step. Awake bitmap = the dilation of the
 // last detector bits, parity bitmap, detector bytes (a tile that did not fire
 // holds 0, the invariant the finalize's dilation relies on), steps, cell
@@ -434,24 +463,28 @@ __global__ __launch_bounds__(kFinNT) void dense_canon_kernel(const Inst* __restr
     const bool valid = t < nt;
     const bool regular = I.mode == 0;
     const int last = n & 1;
-    int f = 0, p = 0, a = 0, r = 0;
-    if (valid) {
-        f = I.dF[last][t];
-        p = I.dP[last][t];
-        r = I.druns[t];
-        const int tj = t / ntx, ti = t - tj * ntx;
-        a = regular ? 1 : dilated2d(I.dF[last], ti, tj, ntx, nb2_of(ti, tj, ntx, I.nty, I.bc_x, I.bc_y), 0);
-    }
-    const uint64_t wa = __ballot(a), wp = __ballot(p);
+    const int nsets = I.nsets;
     const int nw = (nt + 63) >> 6;
-    if (lane == 0 && wid < nw) {
-        I.par[0][wid] = wp;
-        if (!regular) I.a_cur[0][wid] = wa;
+    const int r = valid ? I.druns[t] : 0;
+    uint64_t wa = 0ull;
+    for (int s = 0; s < nsets; s++) {
+        int f = 0, p = 0, a = 0;
+        if (valid) {
+            f = I.dF[last][s * nt + t];
+            p = I.dP[last][s * nt + t];
+            const int tj = t / ntx, ti = t - tj * ntx;
+            a = regular ? 1
+                        : dilated2d(I.dF[last] + s * nt, ti, tj, ntx, nb2_of(ti, tj, ntx, I.nty, I.bc_x, I.bc_y), 0);
+            I.nflag[s][t] = (uint8_t)f;
+        }
+        const uint64_t ws = __ballot(a), wp = __ballot(p);
+        wa |= ws;
+        if (lane == 0 && wid < nw) {
+            I.par[s][wid] = wp;
+            if (!regular) I.a_cur[s][wid] = ws;
+        }
     }
-    if (valid) {
-        I.nflag[0][t] = (uint8_t)f;
-        I.druns[t] = 0;
-    }
+    if (valid) I.druns[t] = 0;
     unsigned long long cells = valid ? (unsigned long long)r * (unsigned long long)(I.tx * I.ty * I.tz) : 0ull;
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) cells += __shfl_down(cells, off, 64);

@@ -15,6 +15,7 @@
 // eps eps' grad phi on a 1-cell ring), T with a 1-cell halo.
 #pragma once
 #include "common.h"
+#include "k_finalize.h"
 
 namespace dba {
 
@@ -41,116 +42,176 @@ __device__ __forceinline__ void k93_aniso(CInst& I, double gx, double gy, double
     *a = e * ep;
 }
 
-template <int TB, int NT>
-__global__ __launch_bounds__(NT) void k93_step_kernel(const WorkList wl) {
+// LDS of one K93 tile.
+template <int TB>
+struct K93Lds {
+    static constexpr int W2 = TB + 4;
+    static constexpr int W1 = TB + 2;
+    double sp[W2 * W2];
+    double sT[W1 * W1];
+    double sFx[W1 * W1];
+    double sFy[W1 * W1];
+    double sE2[TB * TB];
+};
+
+// One tile's step (two sets: p, T). WORKLIST: fill the neighbour table from the
+// record and store the detector bytes here; otherwise the caller has filled the
+// table and stores the returned bits (p) | (T) << 1 (uniform over the
+// workgroup, 0 in regular mode; bit s is meaningful only for a set that ran).
+template <int TB, int NT, bool WORKLIST>
+__device__ __forceinline__ int k93_tile(CInst& I, int t, uint64_t rec, NbTable& nb, K93Lds<TB>& L,
+                                        int* sflag) {
     constexpr int W2 = TB + 4;
     constexpr int W1 = TB + 2;
-    __shared__ double sp[W2 * W2];
-    __shared__ double sT[W1 * W1];
-    __shared__ double sFx[W1 * W1];
-    __shared__ double sFy[W1 * W1];
-    __shared__ double sE2[TB * TB];
+    double* sp = L.sp;
+    double* sT = L.sT;
+    double* sFx = L.sFx;
+    double* sFy = L.sFy;
+    double* sE2 = L.sE2;
+    const int tid = threadIdx.x;
+    const int nx = I.nx, ny = I.ny, ntx = I.ntx, bcx = I.bc_x, bcy = I.bc_y;
+    const int x0 = (t % ntx) * TB, y0 = (t / ntx) * TB;
+    const bool regular = I.mode == 0;
+    const gdouble* __restrict__ p0 = I.buf[0][0];
+    const gdouble* __restrict__ p1 = I.buf[1][0];
+    const gdouble* __restrict__ T0 = I.buf[0][1];
+    const gdouble* __restrict__ T1 = I.buf[1][1];
+    if (WORKLIST) {
+        nb_fill<TB>(nb, I, t, x0, y0, regular, 2, 0, rec);
+        __syncthreads();
+    }
+    const int ap = nb.act[0][4], at = nb.act[1][4];
+    const int pp0 = nb.par[0][4], pp1 = nb.par[1][4];
+    gdouble* __restrict__ pnew = pp0 ? I.buf[0][0] : I.buf[1][0];
+    gdouble* __restrict__ Tnew = pp1 ? I.buf[0][1] : I.buf[1][1];
+
+    // stage phi (2-cell halo) when p-awake, T (1-cell halo) when T-awake,
+    // and phi / T of the tile itself always (the T update needs phi' - phi)
+    for (int k = tid; k < W2 * W2; k += NT) {
+        const int lx = k % W2, ly = k / W2;
+        const int d = nb_dir1<TB, 2>(lx) + 3 * nb_dir1<TB, 2>(ly);
+        if (d != 4 && !ap) continue;
+        const int mx = map_index(x0 + lx - 2, nx, bcx), my = map_index(y0 + ly - 2, ny, bcy);
+        sp[k] = (nb.par[0][d] ? p1 : p0)[mx + (int64_t)my * nx];
+    }
+    for (int k = tid; k < W1 * W1; k += NT) {
+        const int lx = k % W1, ly = k / W1;
+        const int d = nb_dir1<TB, 1>(lx) + 3 * nb_dir1<TB, 1>(ly);
+        if (d != 4 && !at) continue;
+        const int mx = map_index(x0 + lx - 1, nx, bcx), my = map_index(y0 + ly - 1, ny, bcy);
+        sT[k] = (nb.par[1][d] ? T1 : T0)[mx + (int64_t)my * nx];
+    }
+    __syncthreads();
+
+    // eps eps' grad phi on the tile + ring, eps^2 on the tile
+    if (ap) {
+        const double inv2dx = I.inv2dx;
+        for (int k = tid; k < W1 * W1; k += NT) {
+            const int lx = k % W1, ly = k / W1;
+            const int cx = lx + 1, cy = ly + 1;
+            const double gx = (sp[(cx + 1) + cy * W2] - sp[(cx - 1) + cy * W2]) * inv2dx;
+            const double gy = (sp[cx + (cy + 1) * W2] - sp[cx + (cy - 1) * W2]) * inv2dx;
+            double e, a;
+            k93_aniso(I, gx, gy, &e, &a);
+            sFx[k] = a * gx;
+            sFy[k] = a * gy;
+            if (lx >= 1 && lx <= TB && ly >= 1 && ly <= TB) sE2[(lx - 1) + (ly - 1) * TB] = e * e;
+        }
+    }
+    __syncthreads();
+
+    int fp = 0, fT = 0;
+    const double inv2dx = I.inv2dx, invdx2 = I.invdx2, alpha_pi = I.alpha_pi, gam = I.gamma;
+    const double t_eq = I.t_eq, dt_tau = I.dt_tau, dt = I.dt, lk = I.latent_k;
+    const double mp = I.thresh_a, mT = I.thresh_b;
+    for (int k = tid; k < TB * TB; k += NT) {
+        const int lx = k % TB, ly = k / TB;
+        const int64_t gi = (x0 + lx) + (int64_t)(y0 + ly) * nx;
+        const double ph = sp[(lx + 2) + (ly + 2) * W2];
+        const double Tv = sT[(lx + 1) + (ly + 1) * W1];
+        double pn = ph;
+        if (ap) {
+            const int cx = lx + 2, cy = ly + 2, fx = lx + 1, fy = ly + 1;
+            const double cross = ((sFx[fx + (fy + 1) * W1] - sFx[fx + (fy - 1) * W1]) -
+                                  (sFy[(fx + 1) + fy * W1] - sFy[(fx - 1) + fy * W1])) * inv2dx;
+            const double lap = lds_lap5<W2>(sp, cx, cy, invdx2);
+            const double m = alpha_pi * atan(gam * (t_eq - Tv));
+            const double rhs = (cross + sE2[k] * lap) + (ph * (1.0 - ph)) * ((ph - 0.5) + m);
+            pn = ph + dt_tau * rhs;
+            pnew[gi] = pn;
+            fp |= (pn >= mp && pn <= (1.0 - mp));
+        }
+        if (at) {
+            const double lapT = lds_lap5<W1>(sT, lx + 1, ly + 1, invdx2);
+            const double t1 = Tv + dt * lapT;
+            const double Tn = ap ? t1 + lk * (pn - ph) : t1;
+            Tnew[gi] = Tn;
+            fT |= fabs(Tn - Tv) >= mT;
+        }
+    }
+    int bits = 0;
+    if (!regular) {
+        const int anyp = block_any<NT>(fp, sflag);
+        const int anyT = block_any<NT>(fT, sflag);
+        bits = anyp | anyT << 1;
+        if (WORKLIST && tid == 0) {
+            if (ap) I.nflag[0][t] = (uint8_t)anyp;
+            if (at) I.nflag[1][t] = (uint8_t)anyT;
+        }
+    }
+    __syncthreads();
+    return bits;
+}
+
+template <int TB, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k93_step_kernel(const WorkList wl) {
+    __shared__ K93Lds<TB> L;
     __shared__ int sflag[NT / 64 + 1];
     __shared__ NbTable nb;
 
     WorkCursor cur;
     work_begin(wl, cur, true);
     const int total = wl.offsets[wl.nseg];
-    const int tid = threadIdx.x;
     for (;;) {
         CInst* Ip;
         int t;
         uint64_t rec;
         if (!work_next(wl, cur, total, &Ip, &t, &rec)) break;
-        CInst& I = *Ip;
-        const int nx = I.nx, ny = I.ny, ntx = I.ntx, bcx = I.bc_x, bcy = I.bc_y;
-        const int x0 = (t % ntx) * TB, y0 = (t / ntx) * TB;
-        const bool regular = I.mode == 0;
-        const gdouble* __restrict__ p0 = I.buf[0][0];
-        const gdouble* __restrict__ p1 = I.buf[1][0];
-        const gdouble* __restrict__ T0 = I.buf[0][1];
-        const gdouble* __restrict__ T1 = I.buf[1][1];
-        nb_fill<TB>(nb, I, t, x0, y0, regular, 2, 0, rec);
-        __syncthreads();
-        const int ap = nb.act[0][4], at = nb.act[1][4];
-        const int pp0 = nb.par[0][4], pp1 = nb.par[1][4];
-        gdouble* __restrict__ pnew = pp0 ? I.buf[0][0] : I.buf[1][0];
-        gdouble* __restrict__ Tnew = pp1 ? I.buf[0][1] : I.buf[1][1];
+        k93_tile<TB, NT, true>(*Ip, t, rec, nb, L, sflag);
+    }
+}
 
-        // stage phi (2-cell halo) when p-awake, T (1-cell halo) when T-awake,
-        // and phi / T of the tile itself always (the T update needs phi' - phi)
-        for (int k = tid; k < W2 * W2; k += NT) {
-            const int lx = k % W2, ly = k / W2;
-            const int d = nb_dir1<TB, 2>(lx) + 3 * nb_dir1<TB, 2>(ly);
-            if (d != 4 && !ap) continue;
-            const int mx = map_index(x0 + lx - 2, nx, bcx), my = map_index(y0 + ly - 2, ny, bcy);
-            sp[k] = (nb.par[0][d] ? p1 : p0)[mx + (int64_t)my * nx];
-        }
-        for (int k = tid; k < W1 * W1; k += NT) {
-            const int lx = k % W1, ly = k / W1;
-            const int d = nb_dir1<TB, 1>(lx) + 3 * nb_dir1<TB, 1>(ly);
-            if (d != 4 && !at) continue;
-            const int mx = map_index(x0 + lx - 1, nx, bcx), my = map_index(y0 + ly - 1, ny, bcy);
-            sT[k] = (nb.par[1][d] ? T1 : T0)[mx + (int64_t)my * nx];
-        }
-        __syncthreads();
-
-        // eps eps' grad phi on the tile + ring, eps^2 on the tile
-        if (ap) {
-            const double inv2dx = I.inv2dx;
-            for (int k = tid; k < W1 * W1; k += NT) {
-                const int lx = k % W1, ly = k / W1;
-                const int cx = lx + 1, cy = ly + 1;
-                const double gx = (sp[(cx + 1) + cy * W2] - sp[(cx - 1) + cy * W2]) * inv2dx;
-                const double gy = (sp[cx + (cy + 1) * W2] - sp[cx + (cy - 1) * W2]) * inv2dx;
-                double e, a;
-                k93_aniso(I, gx, gy, &e, &a);
-                sFx[k] = a * gx;
-                sFy[k] = a * gy;
-                if (lx >= 1 && lx <= TB && ly >= 1 && ly <= TB) sE2[(lx - 1) + (ly - 1) * TB] = e * e;
+// Single-launch step of a single-instance K93 group (see ch_dense_kernel): per
+// set, awake bits and parities of the 3x3 owner tiles from the bitmaps (first
+// step) or the previous step's bytes; the tile runs when either set is awake.
+template <int TB, int NT>
+__global__ __launch_bounds__(NT) void k93_dense_kernel(const Inst* __restrict__ insts, int first, int s) {
+    __shared__ K93Lds<TB> L;
+    __shared__ int sflag[NT / 64 + 1];
+    __shared__ NbTable nb;
+    CInst& I = ((CInst*)insts)[0];
+    const int nt = I.nt;
+    const int t = xcd_deal(blockIdx.x, gridDim.x, 0, gridDim.x);
+    const int prev = s & 1, next = prev ^ 1;
+    const bool regular = I.mode == 0;
+    dense_nb_fill(nb, I, t, first, prev, regular, 2);
+    __syncthreads();
+    const int ap = nb.act[0][4], at = nb.act[1][4];
+    int bits = 0;
+    if (ap | at) bits = k93_tile<TB, NT, false>(I, t, 0, nb, L, sflag);
+    if (threadIdx.x == 0) {
+        I.dF[next][t] = (uint8_t)(ap ? bits & 1 : 0);
+        I.dF[next][nt + t] = (uint8_t)(at ? bits >> 1 : 0);
+        I.dP[next][t] = (uint8_t)(nb.par[0][4] ^ ap);
+        I.dP[next][nt + t] = (uint8_t)(nb.par[1][4] ^ at);
+        if (ap | at) {
+            I.druns[t] += 1;
+            const int64_t step = I.ctr[0] + s;
+            if (step < I.hist_cap) {
+                if (ap) atomicAdd((int*)&I.hist[2 * step], 1);
+                if (at) atomicAdd((int*)&I.hist[2 * step + 1], 1);
             }
         }
-        __syncthreads();
-
-        int fp = 0, fT = 0;
-        const double inv2dx = I.inv2dx, invdx2 = I.invdx2, alpha_pi = I.alpha_pi, gam = I.gamma;
-        const double t_eq = I.t_eq, dt_tau = I.dt_tau, dt = I.dt, lk = I.latent_k;
-        const double mp = I.thresh_a, mT = I.thresh_b;
-        for (int k = tid; k < TB * TB; k += NT) {
-            const int lx = k % TB, ly = k / TB;
-            const int64_t gi = (x0 + lx) + (int64_t)(y0 + ly) * nx;
-            const double ph = sp[(lx + 2) + (ly + 2) * W2];
-            const double Tv = sT[(lx + 1) + (ly + 1) * W1];
-            double pn = ph;
-            if (ap) {
-                const int cx = lx + 2, cy = ly + 2, fx = lx + 1, fy = ly + 1;
-                const double cross = ((sFx[fx + (fy + 1) * W1] - sFx[fx + (fy - 1) * W1]) -
-                                      (sFy[(fx + 1) + fy * W1] - sFy[(fx - 1) + fy * W1])) * inv2dx;
-                const double lap = lds_lap5<W2>(sp, cx, cy, invdx2);
-                const double m = alpha_pi * atan(gam * (t_eq - Tv));
-                const double rhs = (cross + sE2[k] * lap) + (ph * (1.0 - ph)) * ((ph - 0.5) + m);
-                pn = ph + dt_tau * rhs;
-                pnew[gi] = pn;
-                fp |= (pn >= mp && pn <= (1.0 - mp));
-            }
-            if (at) {
-                const double lapT = lds_lap5<W1>(sT, lx + 1, ly + 1, invdx2);
-                const double t1 = Tv + dt * lapT;
-                const double Tn = ap ? t1 + lk * (pn - ph) : t1;
-                Tnew[gi] = Tn;
-                fT |= fabs(Tn - Tv) >= mT;
-            }
-        }
-
-        if (!regular) {
-            const int anyp = block_any<NT>(fp, sflag);
-            const int anyT = block_any<NT>(fT, sflag);
-            if (tid == 0) {
-                if (ap) I.nflag[0][t] = (uint8_t)anyp;
-                if (at) I.nflag[1][t] = (uint8_t)anyT;
-            }
-        }
-        __syncthreads();
     }
 }
 

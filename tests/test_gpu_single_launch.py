@@ -63,6 +63,46 @@ def test_single_launch_equals_worklist(ctx, tile, stencil, bc, mode):
         assert a["masks"][0].sum() < a["masks"][0].size
 
 
+def k93_run(ctx, p, phi, T, single, plan):
+    ctx.set_single_launch(single)
+    inst = ctx.create(p)
+    inst.upload(abi.F_PHI, phi)
+    inst.upload(abi.F_T, T)
+    masks = []
+    for steps, chunk in plan:
+        inst.step(steps, graph_chunk=chunk)
+        masks.append((inst.active(0)[1], inst.active(1)[1]))
+    info = inst.info()
+    out = dict(phi=inst.download(0), T=inst.download(1), masks=masks, steps=info.steps_done,
+               updates=info.cell_updates, time=info.sim_time, hist=inst.history(), work=inst.worklist(),
+               single=info.single_launch)
+    inst.destroy()
+    ctx.set_single_launch(True)
+    return out
+
+
+@pytest.mark.parametrize("tile,mode", [(16, abi.MODE_DBA), (8, abi.MODE_DBA), (32, abi.MODE_DBA),
+                                       (16, abi.MODE_REGULAR)])
+def test_k93_single_launch_equals_worklist(ctx, tile, mode):
+    n = 256
+    p = configs.dendrite_k93_params(n=n, tile=tile, mode=mode, history_cap=400)
+    phi, T = configs.dendrite_k93_ic(n, n, seeds=[(90, 100), (170, 150)])
+    plan = [(1, 0), (9, 0), (150, 50), (33, 10), (2, 0)]
+    a = k93_run(ctx, p, phi, T, False, plan)
+    b = k93_run(ctx, p, phi, T, True, plan)
+    assert a["single"] == 0 and b["single"] == 1
+    assert np.array_equal(a["phi"], b["phi"]) and np.array_equal(a["T"], b["T"])
+    for (a0, a1), (b0, b1) in zip(a["masks"], b["masks"]):
+        assert np.array_equal(a0, b0) and np.array_equal(a1, b1)
+    assert a["steps"] == b["steps"] and a["updates"] == b["updates"] and a["time"] == b["time"]
+    assert np.array_equal(a["hist"], b["hist"])
+    assert np.array_equal(a["work"], b["work"])
+    if mode == abi.MODE_DBA:
+        p_awake, t_awake = a["masks"][-1]
+        assert 0 < p_awake.sum() < p_awake.size and 0 < t_awake.sum() < t_awake.size
+        assert not np.array_equal(p_awake, t_awake)   # the two sets differ
+
+
 def test_conserved_and_batched_use_the_worklist_path(ctx):
     """Conserved DBA and multi-instance groups are not single-launch eligible;
     they run (and match the oracle elsewhere) with the switch on."""
