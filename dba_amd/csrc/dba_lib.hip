@@ -421,7 +421,9 @@ dba_status get_group(dba_ctx* ctx, const std::vector<int>& ids, Group** out) {
     g->tile = first->p.tile_x;
     g->tile_z = first->p.tile_z;
     g->threads = first->p.threads ? first->p.threads : default_threads(g->model, g->tile);
-    int64_t cap = (g->model == DBA_MODEL_DENDRITE_K94 || g->model == DBA_MODEL_MODELB) ? 16384 : 4096;
+    // Model B: 65536 workgroups (4 blocks each on a fully awake 256^3 field)
+    // measured faster than 16384 or 3072 with longer pipelines per wave
+    int64_t cap = g->model == DBA_MODEL_MODELB ? 65536 : (g->model == DBA_MODEL_DENDRITE_K94 ? 16384 : 4096);
     g->grid = (int)std::max<int64_t>(1, std::min<int64_t>(g->total_tiles, cap));
     size_t n = ids.size();
     HIP_TRY(hipMalloc(&g->d_insts, sizeof(Inst) * n));

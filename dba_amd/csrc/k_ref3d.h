@@ -50,11 +50,11 @@ __device__ __forceinline__ int wrapi(int i, int n) {
     return i;
 }
 
-// 27-point isotropic Laplacian from an LDS box of extents WX x WY x WZ, in the
-// summation order of example_Reactive_ModelB/kernel.cu:333-344.
-template <int WX, int WY>
+// 27-point isotropic Laplacian from an LDS box with row stride RX and plane
+// stride PZ, in the summation order of example_Reactive_ModelB/kernel.cu:333-344.
+template <int RX, int PZ>
 __device__ __forceinline__ double lds_lap27(const double* s, int x, int y, int z, double inv_dx2) {
-#define S3(a, b, c) s[(x + (a)) + (y + (b)) * WX + (z + (c)) * WX * WY]
+#define S3(a, b, c) s[(x + (a)) + (y + (b)) * RX + (z + (c)) * PZ]
     return inv_dx2 *
            (-64.0 / 15.0 * S3(0, 0, 0) +
             7.0 / 15.0 * (S3(1, 0, 0) + S3(-1, 0, 0) + S3(0, 1, 0) + S3(0, -1, 0) + S3(0, 0, 1) + S3(0, 0, -1)) +
@@ -92,6 +92,7 @@ __device__ __forceinline__ int nb_dir3(int lx, int ly, int lz) {
 template <int TB, int TZ>
 struct Box1 {
     static constexpr int WX = TB + 2, WZ = TZ + 2, A = WX * WX * WZ, NH = (A + 63) / 64;
+    static constexpr int RX = WX, PZ = WX * WX, AP = A;  // LDS row and plane strides
     int c[NH];
     __device__ __forceinline__ void init(int lane) {
 #pragma unroll
@@ -107,6 +108,49 @@ struct Box1 {
 __device__ __forceinline__ int wrap1(int v, int n) {
     v = v < 0 ? v + n : v;
     return v >= n ? v - n : v;
+}
+
+// Band sweep, for a single instance whose list holds every block (the usual
+// case of a fully awake 3D field). Workgroups are dealt to the 8 XCDs
+// round-robin by index, so XCD x = b mod 8 owns the band of Cx = ntx*nty/8
+// consecutive block columns [x Cx, (x+1) Cx) and sweeps it z-layer by z-layer:
+// the band's blocks in (z-block, column) order are dealt round-robin to the
+// XCD's S = G/8 workgroups (item i = b/8 + k S). With a grid that is resident
+// at once, the XCD's workgroups work on a contiguous run of that order at any
+// moment, so a halo plane is still in the XCD's L2 when the blocks on its
+// other side need it, and only the two band edges read across XCDs.
+// cx = 0 where the grid or the list does not allow it (list order applies).
+struct BandSweep {
+    int cx, c, per, s, j, base;
+};
+
+__device__ __forceinline__ BandSweep band_sweep_setup(const WorkList& w, int total) {
+    BandSweep bs{0, 0, 0, 0, 0, 0};
+    // one instance (segments are in instance order) whose list is full
+    if (w.nseg < 1 || w.segs[w.nseg - 1].inst != 0) return bs;
+    const CInst& I = *(const CInst*)w.insts;
+    if (total != I.nt) return bs;
+    const int G = gridDim.x, C = I.ntx * I.nty;
+    if ((G & 7) != 0 || (C & 7) != 0) return bs;
+    bs.cx = C >> 3;
+    bs.c = C;
+    bs.per = I.nt >> 3;
+    bs.s = G >> 3;
+    bs.j = blockIdx.x >> 3;
+    bs.base = (blockIdx.x & 7) * bs.cx;
+    return bs;
+}
+
+// The sweep's blocks k = round0 .. round0+63 of this workgroup in the RoundList
+// form (lane k - round0: item b/8 + k S of its XCD's order).
+__device__ __forceinline__ RoundList band_sweep_round(const BandSweep& bs, int round0) {
+    const int64_t i = bs.j + (int64_t)(round0 + (int)(threadIdx.x & 63)) * bs.s;
+    const bool ok = i < bs.per;
+    RoundList L;
+    L.inst = 0;
+    L.t = ok ? (int)(i / bs.cx) * bs.c + bs.base + (int)(i % bs.cx) : 0;
+    L.valid = __ballot(ok);
+    return L;
 }
 
 // A block of the worklist and what its loads need (all wave-uniform).
@@ -181,9 +225,9 @@ __device__ __forceinline__ int64_t mb_own_index(const MbBlock& b, int lane) {
 template <int TB, int TZ>
 __global__ __launch_bounds__(64) void modelb_mu_kernel(const WorkList wl) {
     using B1 = Box1<TB, TZ>;
-    constexpr int WX = B1::WX, A = B1::A, NH = B1::NH, C = TB * TB * TZ;
+    constexpr int A = B1::A, NH = B1::NH, C = TB * TB * TZ, RX = B1::RX, PZ = B1::PZ, AP = B1::AP;
     static_assert(C <= 64, "at most one cell per lane");
-    __shared__ double sc[3 * A];
+    __shared__ double sc[3 * AP];
     __shared__ double sshift[2][kMaxFields][27];
 
     const int lane = threadIdx.x;
@@ -206,8 +250,9 @@ __global__ __launch_bounds__(64) void modelb_mu_kernel(const WorkList wl) {
     };
 
     const int total = wl.offsets[wl.nseg];
+    const BandSweep bs = band_sweep_setup(wl, total);
     for (int round0 = 0; (int64_t)round0 * gridDim.x < total; round0 += 64) {
-        const RoundList L = round_list(wl, total, round0);
+        const RoundList L = bs.cx ? band_sweep_round(bs, round0) : round_list(wl, total, round0);
         uint64_t todo = L.valid;
         if (!todo) continue;
         MbBlock b = mb_take<TB, TZ>(wl, L, todo);
@@ -235,7 +280,7 @@ __global__ __launch_bounds__(64) void modelb_mu_kernel(const WorkList wl) {
                     for (int f = 0; f < 3; f++) {
                         double v = h[i][f];
                         if (f < ncons) v -= sshift[slot][f][d];
-                        sc[f * A + k] = v;
+                        sc[f * AP + k] = v;
                     }
                 }
             }
@@ -258,19 +303,19 @@ __global__ __launch_bounds__(64) void modelb_mu_kernel(const WorkList wl) {
             if (lane < C) {
                 const int lx = lane % TB, ly = (lane / TB) % TB, lz = lane / (TB * TB);
                 const int cx = lx + 1, cy = ly + 1, cz = lz + 1;
-                const int q = cx + cy * WX + cz * WX * WX;
-                const double X = sc[q], Y = sc[A + q], Z = sc[2 * A + q];
+                const int q = cx + cy * RX + cz * PZ;
+                const double X = sc[q], Y = sc[AP + q], Z = sc[2 * AP + q];
                 const double B = 1.0 - X - Y - Z;
                 // logarithms first, then one Laplacian at a time: left free, the
                 // scheduler hoists all 81 LDS reads above the logarithms (250 VGPRs)
                 const double logB = log_f64(B), logX = log_f64(X), logY = log_f64(Y), logZ = log_f64(Z);
                 __builtin_amdgcn_sched_barrier(0);
                 const double inv_dx2 = I.invdx2;
-                const double lx_ = lds_lap27<WX, WX>(sc, cx, cy, cz, inv_dx2);
+                const double lx_ = lds_lap27<RX, PZ>(sc, cx, cy, cz, inv_dx2);
                 __builtin_amdgcn_sched_barrier(0);
-                const double ly_ = lds_lap27<WX, WX>(sc + A, cx, cy, cz, inv_dx2);
+                const double ly_ = lds_lap27<RX, PZ>(sc + AP, cx, cy, cz, inv_dx2);
                 __builtin_amdgcn_sched_barrier(0);
-                const double lz_ = lds_lap27<WX, WX>(sc + 2 * A, cx, cy, cz, inv_dx2);
+                const double lz_ = lds_lap27<RX, PZ>(sc + 2 * AP, cx, cy, cz, inv_dx2);
                 const double dfx = I.mb_cx + I.chi_xy * Y + I.chi_xz * Z + I.chi_xb * B - I.chi_xb * X -
                                    I.chi_yb * Y - I.chi_zb * Z + logX / I.r_x - logB - I.epsx_sq * lx_;
                 const double dfy = I.mb_cy + I.chi_xy * X + I.chi_yz * Z + I.chi_yb * B - I.chi_xb * X -
@@ -305,9 +350,9 @@ __global__ __launch_bounds__(64) void modelb_mu_kernel(const WorkList wl) {
 template <int TB, int TZ>
 __global__ __launch_bounds__(64) void modelb_update_kernel(const WorkList wl) {
     using B1 = Box1<TB, TZ>;
-    constexpr int WX = B1::WX, A = B1::A, NH = B1::NH, C = TB * TB * TZ;
+    constexpr int A = B1::A, NH = B1::NH, C = TB * TB * TZ, RX = B1::RX, PZ = B1::PZ, AP = B1::AP;
     static_assert(C <= 64, "at most one cell per lane");
-    __shared__ double smu[3 * A];
+    __shared__ double smu[3 * AP];
 
     const int lane = threadIdx.x;
     B1 box;
@@ -332,8 +377,9 @@ __global__ __launch_bounds__(64) void modelb_update_kernel(const WorkList wl) {
     };
 
     const int total = wl.offsets[wl.nseg];
+    const BandSweep bs = band_sweep_setup(wl, total);
     for (int round0 = 0; (int64_t)round0 * gridDim.x < total; round0 += 64) {
-        const RoundList L = round_list(wl, total, round0);
+        const RoundList L = bs.cx ? band_sweep_round(bs, round0) : round_list(wl, total, round0);
         uint64_t todo = L.valid;
         if (!todo) continue;
         MbBlock b = mb_take<TB, TZ>(wl, L, todo);
@@ -352,7 +398,7 @@ __global__ __launch_bounds__(64) void modelb_update_kernel(const WorkList wl) {
             for (int i = 0; i < NH; i++) {
                 const int k = lane + i * 64;
                 if (k < A)
-                    for (int f = 0; f < 3; f++) smu[f * A + k] = h[i][f];
+                    for (int f = 0; f < 3; f++) smu[f * AP + k] = h[i][f];
             }
             __syncthreads();
             double vn[3] = {0.0, 0.0, 0.0}, en[3] = {0.0, 0.0, 0.0};
@@ -378,18 +424,18 @@ __global__ __launch_bounds__(64) void modelb_update_kernel(const WorkList wl) {
                 double x[3] = {v[0], v[1], v[2]};
                 for (int f = 0; f < ncons; f++) x[f] -= I.scal[8 + f] - e[f];
                 const double inv_dx2 = I.invdx2;
-                const int mq = (lx + 1) + (ly + 1) * WX + (lz + 1) * WX * WX;
-                const double dfx = smu[mq], dfy = smu[A + mq], dfz = smu[2 * A + mq];
+                const int mq = (lx + 1) + (ly + 1) * RX + (lz + 1) * PZ;
+                const double dfx = smu[mq], dfy = smu[AP + mq], dfz = smu[2 * AP + mq];
                 const double R = I.k_0 * (exp(I.n_exp * I.v_x * dfx + I.m_exp * I.v_y * dfy) - exp(I.v_z * dfz));
                 __builtin_amdgcn_sched_barrier(0);
                 const double chx =
-                    I.dt * I.v_x * (I.mob_x * lds_lap27<WX, WX>(smu, lx + 1, ly + 1, lz + 1, inv_dx2) - I.n_exp * R);
+                    I.dt * I.v_x * (I.mob_x * lds_lap27<RX, PZ>(smu, lx + 1, ly + 1, lz + 1, inv_dx2) - I.n_exp * R);
                 __builtin_amdgcn_sched_barrier(0);
                 const double chy = I.dt * I.v_y *
-                                   (I.mob_y * lds_lap27<WX, WX>(smu + A, lx + 1, ly + 1, lz + 1, inv_dx2) - I.m_exp * R);
+                                   (I.mob_y * lds_lap27<RX, PZ>(smu + AP, lx + 1, ly + 1, lz + 1, inv_dx2) - I.m_exp * R);
                 __builtin_amdgcn_sched_barrier(0);
                 const double chz =
-                    I.dt * I.v_z * (I.mob_z * lds_lap27<WX, WX>(smu + 2 * A, lx + 1, ly + 1, lz + 1, inv_dx2) + R);
+                    I.dt * I.v_z * (I.mob_z * lds_lap27<RX, PZ>(smu + 2 * AP, lx + 1, ly + 1, lz + 1, inv_dx2) + R);
                 const int64_t gi = mb_own_index<TB, TZ>(b, lane);
                 I.buf[pself ^ 1][0][gi] = x[0] + chx;
                 I.buf[pself ^ 1][1][gi] = x[1] + chy;

@@ -54,6 +54,28 @@ def test_modelb_matches_oracle(ctx, mode, nz):
     inst.destroy()
 
 
+@pytest.mark.parametrize("mode", [abi.MODE_REGULAR, abi.MODE_DBA, abi.MODE_CONSERVED])
+def test_modelb_band_sweep_matches_oracle(ctx, mode):
+    """192 x 192 x 128 holds 73728 blocks, more than the 65536-workgroup grid:
+    with every block awake the list is taken in the band-sweep order (each
+    XCD's band of block columns z-layer by z-layer, some workgroups taking two
+    blocks), the order used for a full single-instance list."""
+    n, nz = 192, 128
+    X = list(configs.modelb_ic(n, n, nz)) + [np.zeros((nz, n, n))]
+    p = configs.modelb_params(n=n, nz=nz, mode=mode, eps=1e-4)
+    inst, o = pair(ctx, p, X)
+    for chunk in (1, 2):
+        inst.step(chunk)
+        o.step(chunk)
+        for f in range(3):
+            d = np.max(np.abs(inst.download(f) - o.get_field(f)))
+            assert d <= TOL, (f, d)
+        if mode != abi.MODE_REGULAR:
+            assert np.array_equal(inst.active(0)[1], o.awake(0))
+            assert inst.active(0)[0] == inst.info().ntiles   # the full list
+    inst.destroy()
+
+
 @pytest.mark.parametrize("mode", [abi.MODE_REGULAR, abi.MODE_DBA])
 @pytest.mark.parametrize("nz", [32, 1])
 def test_k94_matches_oracle(ctx, mode, nz):
