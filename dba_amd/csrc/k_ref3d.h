@@ -214,6 +214,27 @@ __device__ __forceinline__ int64_t mb_box_index(const MbBlock& b, int c) {
            (int64_t)wrap1(b.z0 + lz - 1, I.nz) * ((int64_t)I.nx * I.ny);
 }
 
+// A block whose 1-cell box needs no periodic wrap addresses the box as one
+// uniform base plus a per-lane offset (the same for every such block of the
+// instance): no wrap selects and no 64-bit index arithmetic per element.
+template <int TB, int TZ>
+__device__ __forceinline__ bool mb_interior(const MbBlock& b) {
+    CInst& I = *b.I;
+    return b.x0 > 0 && b.x0 + TB < I.nx && b.y0 > 0 && b.y0 + TB < I.ny && b.z0 > 0 && b.z0 + TZ < I.nz &&
+           (int64_t)I.nx * I.ny < (1 << 28);
+}
+
+// uniform index of the box corner (x0-1, y0-1, z0-1) of an interior block
+__device__ __forceinline__ int64_t mb_box_base(const MbBlock& b) {
+    CInst& I = *b.I;
+    return (b.x0 - 1) + (int64_t)(b.y0 - 1) * I.nx + (int64_t)(b.z0 - 1) * ((int64_t)I.nx * I.ny);
+}
+
+// per-lane offset of packed box element c from the box corner
+__device__ __forceinline__ int mb_box_offset(int c, int nx, int nxy) {
+    return (c & 15) + ((c >> 4) & 15) * nx + ((c >> 8) & 15) * nxy;
+}
+
 template <int TB, int TZ>
 __device__ __forceinline__ int64_t mb_own_index(const MbBlock& b, int lane) {
     CInst& I = *b.I;
@@ -235,15 +256,33 @@ __global__ __launch_bounds__(64) void modelb_mu_kernel(const WorkList wl) {
     box.init(lane);
     double h[NH][3];
     // block q's box (parities known) into h, its own phi into ph
+    constexpr uint64_t kAll27 = (1ull << 27) - 1;
     auto load = [&](const MbBlock& q, double& ph) {
         CInst& I = *q.I;
+        if (mb_interior<TB, TZ>(q) && (q.pm == 0 || q.pm == kAll27)) {
+            // no wrap, and the 27 blocks current in one buffer
+            const int p = (int)(q.pm & 1);
+            const int64_t base = mb_box_base(q);
+            const int nx = I.nx, nxy = I.nx * I.ny;
+            const double *b0 = I.buf[p][0] + base, *b1 = I.buf[p][1] + base, *b2 = I.buf[p][2] + base;
 #pragma unroll
-        for (int i = 0; i < NH; i++) {
-            if (lane + i * 64 < A) {
-                const int c = box.c[i];
-                const int64_t gi = mb_box_index<TB, TZ>(q, c);
-                const int pp = (int)(q.pm >> (c >> 12)) & 1;
-                for (int f = 0; f < 3; f++) h[i][f] = I.buf[pp][f][gi];
+            for (int i = 0; i < NH; i++) {
+                if (lane + i * 64 < A) {
+                    const int o = mb_box_offset(box.c[i], nx, nxy);
+                    h[i][0] = b0[o];
+                    h[i][1] = b1[o];
+                    h[i][2] = b2[o];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < NH; i++) {
+                if (lane + i * 64 < A) {
+                    const int c = box.c[i];
+                    const int64_t gi = mb_box_index<TB, TZ>(q, c);
+                    const int pp = (int)(q.pm >> (c >> 12)) & 1;
+                    for (int f = 0; f < 3; f++) h[i][f] = I.buf[pp][f][gi];
+                }
             }
         }
         if (lane < C) ph = I.buf[(int)(q.pm >> 13) & 1][3][mb_own_index<TB, TZ>(q, lane)];
@@ -362,11 +401,26 @@ __global__ __launch_bounds__(64) void modelb_update_kernel(const WorkList wl) {
     // their shift epochs into eq (conserved)
     auto load = [&](const MbBlock& q, int pq, double (&vq)[3], double (&eq)[3]) {
         CInst& I = *q.I;
+        if (mb_interior<TB, TZ>(q)) {
+            const int64_t base = mb_box_base(q);
+            const int nx = I.nx, nxy = I.nx * I.ny;
+            const double *a0 = I.aux[0] + base, *a1 = I.aux[1] + base, *a2 = I.aux[2] + base;
 #pragma unroll
-        for (int i = 0; i < NH; i++) {
-            if (lane + i * 64 < A) {
-                const int64_t gi = mb_box_index<TB, TZ>(q, box.c[i]);
-                for (int f = 0; f < 3; f++) h[i][f] = I.aux[f][gi];
+            for (int i = 0; i < NH; i++) {
+                if (lane + i * 64 < A) {
+                    const int o = mb_box_offset(box.c[i], nx, nxy);
+                    h[i][0] = a0[o];
+                    h[i][1] = a1[o];
+                    h[i][2] = a2[o];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < NH; i++) {
+                if (lane + i * 64 < A) {
+                    const int64_t gi = mb_box_index<TB, TZ>(q, box.c[i]);
+                    for (int f = 0; f < 3; f++) h[i][f] = I.aux[f][gi];
+                }
             }
         }
         if (lane < C) {

@@ -26,10 +26,12 @@ def pair(ctx, p, fields):
 
 
 @pytest.mark.parametrize("mode", [abi.MODE_REGULAR, abi.MODE_DBA, abi.MODE_CONSERVED])
-@pytest.mark.parametrize("nz", [8, 1])
+@pytest.mark.parametrize("nz", [8, 1, 24])
 def test_modelb_matches_oracle(ctx, mode, nz):
     # 48 wide: the uniform half keeps tile columns that neither the interface
-    # nor its periodic image wakes, so DBA modes have sleeping tiles
+    # nor its periodic image wakes, so DBA modes have sleeping tiles; nz = 24
+    # has blocks whose box needs no wrap (the direct-offset loads), next to
+    # sleeping neighbours whose current buffer differs
     n = 48
     X = list(configs.modelb_ic(n, n, nz)) + [np.zeros((nz, n, n))]
     if mode != abi.MODE_REGULAR:
