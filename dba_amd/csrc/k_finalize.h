@@ -352,11 +352,21 @@ __device__ __forceinline__ void finalize_merge_body(const Inst* __restrict__ ins
                                                     const int32_t* __restrict__ counts,
                                                     int32_t* __restrict__ offsets, int rotate, int* si) {
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    // the offsets scan's first counts, loaded with the reductions' inputs
+    const int v_first = tid < nseg ? counts[tid] : 0;
     for (int i0 = 0; i0 < n_inst; i0 += NT / 64) {
         const int i = i0 + wid;
         if (i >= n_inst) break;
         CInst& J = ((CInst*)insts)[i];
-        const InstSegs r = isegs[i];
+        const InstSegs r = n_inst == 1 ? InstSegs{0, nseg} : isegs[i];
+        // the instance's scalars and counters, loaded ahead of the reduction
+        double sc[16];
+        int64_t step = 0, upd = 0;
+        if (lane == 0) {
+            for (int kk = 0; kk < 16; kk++) sc[kk] = J.scal[kk];
+            step = J.ctr[0];
+            upd = J.ctr[1];
+        }
         int64_t ll = 0;
         unsigned long long q0 = 0, q1 = 0, qc = 0;
         double cs[kMaxFields] = {0.0, 0.0, 0.0, 0.0};
@@ -381,16 +391,12 @@ __device__ __forceinline__ void finalize_merge_body(const Inst* __restrict__ ins
             for (int f = 0; f < kMaxFields; f++) cs[f] += __shfl_down(cs[f], off, 64);
             vm = fmax(vm, __shfl_down(vm, off, 64));
         }
-        if (lane == 0) {
-            double sc[16];
-            for (int kk = 0; kk < 16; kk++) sc[kk] = J.scal[kk];
-            finish_instance(J, ll, q0, q1, qc, cs, vm, J.ctr[0], J.ctr[1], sc, rotate);
-        }
+        if (lane == 0) finish_instance(J, ll, q0, q1, qc, cs, vm, step, upd, sc, rotate);
     }
     int running = 0;
     for (int b = 0; b < nseg; b += NT) {
         const int i = b + tid;
-        const int v = i < nseg ? counts[i] : 0;
+        const int v = b == 0 ? v_first : (i < nseg ? counts[i] : 0);
         int tot;
         const int ex = block_exclusive_scan<NT>(v, si, &tot);
         if (i < nseg) offsets[i] = running + ex;
