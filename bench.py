@@ -250,12 +250,15 @@ def bench_batch(args, K):
     ic_s = time.perf_counter() - t0
     log = (lambda m: print(m, file=sys.stderr, flush=True))
     with Context(0) as ctx:
-        (tile, nt), table = batch.autotune(ctx, items, trial_steps=args.tune_steps, fields_cache=cache, log=log)
-        log(f"autotune: tile {tile}, threads {nt}")
+        (tile, nt, mode), table = batch.autotune(ctx, items, trial_steps=args.tune_steps, fields_cache=cache,
+                                                 log=log)
+        log(f"autotune: tile {tile}, threads {nt}, single-launch mode {mode}")
         st, insts = batch.run_batched(ctx, items, K, tile=tile, threads=nt, chunk=args.chunk, fields_cache=cache,
-                                      keep=True)
+                                      keep=True, single_launch=mode)
+        ctx.set_single_launch(mode)
         log(f"batched: {st.wall_s:.3f} s, {st.active_rate:.4g} active/s")
         roof = kernel_roofline(ctx, insts, 32, args.timing_steps)
+        ctx.set_single_launch(1)
         finite = all(i.check_finite() for i in insts)
         for i in insts:
             i.destroy()
@@ -274,7 +277,8 @@ def bench_batch(args, K):
         "ms_per_step": st.wall_s * 1e3 / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": "batch: 64 Kobayashi-93 dendrites, sides log-U[256,2048], 5000 steps",
-                   "tile": tile, "threads": nt, "graph_chunk": args.chunk, **batch.summary(items)},
+                   "tile": tile, "threads": nt, "single_launch": mode, "graph_chunk": args.chunk,
+                   **batch.summary(items)},
         "full_grid_updates_per_s": st.full_rate,
         "active_fraction": st.active_updates / st.full_updates,
         "batched_vs_sequential_plain": st.active_rate / plain.active_rate if plain else None,
@@ -288,7 +292,8 @@ def bench_batch(args, K):
             "note": "one instance at a time, graph-replayed chunks of steps, stepping time only"},
         "batched_16x16": {"value": st16.active_rate, "wall_s": st16.wall_s,
                           "vs_sequential_wall": seq.wall_s / st16.wall_s if seq else None},
-        "autotune": [{"tile": a, "threads": b, "ms_per_step": c, "active_rate": d} for a, b, c, d in table],
+        "autotune": [{"tile": a, "threads": b, "single_launch": m, "ms_per_step": c, "active_rate": d}
+                     for a, b, m, c, d in table],
         "ic_host_s": ic_s, "finite": bool(finite), "roofline": roof,
     }
     if not args.no_cpu:
@@ -390,6 +395,8 @@ def main():
     ap.add_argument("--tile", type=int, default=None, help="activation tile edge override")
     ap.add_argument("--threads", type=int, default=0, help="workgroup size override (64/128/256)")
     ap.add_argument("--tune-steps", type=int, default=200, help="batch: timed steps per autotuner candidate")
+    ap.add_argument("--single-launch", type=int, default=1, choices=(0, 1),
+                    help="one-instance workloads: 0 forces the worklist path (step kernel + finalize)")
     ap.add_argument("--requests", type=int, default=300, help="serve: requests in the stream")
     ap.add_argument("--rate", type=float, default=4.0, help="serve: initial arrival rate (1/s)")
     ap.add_argument("--budget-gib", type=float, default=4.0, help="serve: device memory budget for admission")
@@ -418,6 +425,7 @@ def main():
     ncell = p.nx * p.ny * p.nz
 
     with Context(0) as ctx:
+        ctx.set_single_launch(args.single_launch)
         inst = ctx.create(p)
         for f, a in enumerate(fields):
             inst.upload(f, a)

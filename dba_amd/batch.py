@@ -32,6 +32,7 @@ from . import abi, configs
 
 TILES = (8, 16, 32)
 THREADS = (64, 128, 256)
+SINGLE_LAUNCH_MODES = (1, 2)
 
 
 @dataclass
@@ -104,16 +105,19 @@ def _create(ctx, items, tile, threads, fields_cache=None):
     return insts
 
 
-def run_batched(ctx, items, steps, tile=16, threads=0, chunk=100, fields_cache=None, keep=False):
-    """All items in one merged-worklist group; returns (stats, instances or None)."""
+def run_batched(ctx, items, steps, tile=16, threads=0, chunk=100, fields_cache=None, keep=False, single_launch=1):
+    """All items in one group (merged worklist, or with single_launch=2 one
+    launch per step over all their tiles); returns (stats, instances or None)."""
     insts = _create(ctx, items, tile, threads, fields_cache)
     ctx.sync()
     u0 = sum(i.info().cell_updates for i in insts)
+    ctx.set_single_launch(single_launch)
     ctx.marker(0)
     t0 = time.perf_counter()
     ctx.step(insts, steps, graph_chunk=min(chunk, steps) if steps else 0)
     ctx.marker(1)
     ctx.sync()
+    ctx.set_single_launch(1)
     wall = time.perf_counter() - t0
     gpu_ms = ctx.elapsed_ms(0, 1)
     u1 = sum(i.info().cell_updates for i in insts)
@@ -164,8 +168,10 @@ def run_sequential(ctx, items, steps, tile=16, threads=0, chunk=100, fields_cach
 
 
 def autotune(ctx, items, trial_steps=200, warm_steps=20, tiles=TILES, threads=THREADS, chunk=100,
-             fields_cache=None, log=None):
-    """Time every (tile, threads) candidate on the sweep's first steps.
+             fields_cache=None, log=None, modes=SINGLE_LAUNCH_MODES):
+    """Time every (tile, threads, single-launch mode) candidate on the sweep's
+    first steps (mode 1: merged worklist, a step kernel and the finalize per
+    step; mode 2: one launch per step over every tile of the batch).
 
     The objective is wall time per step of the whole batch (the same physical
     simulation finishing sooner), not active cell-updates/s: a larger tile wakes
@@ -173,7 +179,8 @@ def autotune(ctx, items, trial_steps=200, warm_steps=20, tiles=TILES, threads=TH
     more work. Threads never change results (bit-identical); the tile sets the
     activation granularity, so results at different tiles agree only to the
     DBA tolerance of the regular-mode solution.
-    Returns (best (tile, threads), [(tile, threads, ms_per_step, active_rate)]).
+    Threads and the mode never change results (bit-identical).
+    Returns (best (tile, threads, mode), [(tile, threads, mode, ms_per_step, active_rate)]).
     """
     if fields_cache is None:
         fields_cache = {it.index: it.fields() for it in items}
@@ -182,22 +189,25 @@ def autotune(ctx, items, trial_steps=200, warm_steps=20, tiles=TILES, threads=TH
         if any(it.side % tile for it in items):
             continue
         for nt in threads:
-            insts = _create(ctx, items, tile, nt, fields_cache)
-            ctx.step(insts, warm_steps, graph_chunk=min(chunk, warm_steps))
-            ctx.sync()
-            u0 = sum(i.info().cell_updates for i in insts)
-            t0 = time.perf_counter()
-            ctx.step(insts, trial_steps, graph_chunk=min(chunk, trial_steps))
-            ctx.sync()
-            dt = time.perf_counter() - t0
-            u1 = sum(i.info().cell_updates for i in insts)
-            table.append((tile, nt, dt * 1e3 / trial_steps, (u1 - u0) / dt))
-            if log:
-                log(f"autotune tile {tile} threads {nt}: {dt * 1e3 / trial_steps:.4f} ms/step")
-            for i in insts:
-                i.destroy()
-    best = min(table, key=lambda r: r[2])
-    return (best[0], best[1]), table
+            for mode in modes:
+                ctx.set_single_launch(mode)
+                insts = _create(ctx, items, tile, nt, fields_cache)
+                ctx.step(insts, warm_steps, graph_chunk=min(chunk, warm_steps))
+                ctx.sync()
+                u0 = sum(i.info().cell_updates for i in insts)
+                t0 = time.perf_counter()
+                ctx.step(insts, trial_steps, graph_chunk=min(chunk, trial_steps))
+                ctx.sync()
+                dt = time.perf_counter() - t0
+                u1 = sum(i.info().cell_updates for i in insts)
+                table.append((tile, nt, mode, dt * 1e3 / trial_steps, (u1 - u0) / dt))
+                if log:
+                    log(f"autotune tile {tile} threads {nt} mode {mode}: {dt * 1e3 / trial_steps:.4f} ms/step")
+                for i in insts:
+                    i.destroy()
+    ctx.set_single_launch(1)
+    best = min(table, key=lambda r: r[3])
+    return (best[0], best[1], best[2]), table
 
 
 def footprint(ctx, items, tile=16):

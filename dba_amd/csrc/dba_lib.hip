@@ -96,6 +96,7 @@ struct Group {
     SegPart* d_parts = nullptr;
     int32_t* d_counts = nullptr;
     int32_t* d_offsets = nullptr;
+    int32_t* d_dmap = nullptr;    // single-launch steps of several instances or segments: instance << 24 | tile
     int nseg = 0;
     int64_t total_tiles = 0;
     int grid = 0;
@@ -133,7 +134,7 @@ struct dba_ctx {
     double* d_diag = nullptr;      // diagnostics: block partials, results
     int32_t* d_cols = nullptr;     // diagnostics: growth-front columns
     bool timing = false;
-    bool single_launch = true;     // single-launch steps for eligible groups (dba_set_single_launch)
+    int single_launch = 1;         // single-launch steps: 0 off, 1 one-instance groups, 2 also batches
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_step, ev_fin;
     std::vector<hipEvent_t> ev_pool;
     hipEvent_t markers[8] = {};
@@ -293,6 +294,7 @@ void free_group(Group* g) {
     (void)hipFree(g->d_parts);
     (void)hipFree(g->d_counts);
     (void)hipFree(g->d_offsets);
+    (void)hipFree(g->d_dmap);
     delete g;
 }
 
@@ -438,6 +440,20 @@ dba_status get_group(dba_ctx* ctx, const std::vector<int>& ids, Group** out) {
     HIP_TRY(hipMalloc(&g->d_counts, sizeof(int32_t) * ns));
     HIP_TRY(hipMalloc(&g->d_offsets, sizeof(int32_t) * (ns + 1)));
     HIP_TRY(hipMemsetAsync(g->d_offsets, 0, sizeof(int32_t) * (ns + 1), ctx->setup));
+    std::vector<int32_t> dmap;
+    if ((n > 1 || ns > 1) && (g->model == DBA_MODEL_CH || g->model == DBA_MODEL_DENDRITE_K93)) {
+        // dense index -> instance << 24 | tile (groups hold at most 255 instances)
+        bool fits = true;
+        for (size_t k = 0; k < n; k++) fits = fits && hs[k].nt < (1 << 24);
+        if (fits) {
+            dmap.reserve((size_t)g->total_tiles);
+            for (size_t k = 0; k < n; k++)
+                for (int t = 0; t < hs[k].nt; t++) dmap.push_back((int32_t)((uint32_t)k << 24 | (uint32_t)t));
+            HIP_TRY(hipMalloc(&g->d_dmap, sizeof(int32_t) * dmap.size()));
+            HIP_TRY(hipMemcpyAsync(g->d_dmap, dmap.data(), sizeof(int32_t) * dmap.size(), hipMemcpyHostToDevice,
+                                   ctx->setup));
+        }
+    }
     // the compute stream is used only after this host-side wait
     HIP_TRY(hipStreamSynchronize(ctx->setup));
     ctx->groups[ids] = g;
@@ -514,14 +530,19 @@ void launch_step(const Group* g, hipStream_t s) {
 // Single-launch steps: one instance in one segment (its whole state fits one
 // canonicalising workgroup), a model with a single-launch kernel, no conserved
 // shift (its per-step global sum needs the finalize).
-bool single_launch_params(const dba_params& p, int nt) {
-    return nt <= kSegTiles && (p.model == DBA_MODEL_CH || p.model == DBA_MODEL_DENDRITE_K93) &&
-           p.mode != DBA_MODE_CONSERVED;
+bool single_launch_params(const dba_params& p) {
+    return (p.model == DBA_MODEL_CH || p.model == DBA_MODEL_DENDRITE_K93) && p.mode != DBA_MODE_CONSERVED;
 }
 
+// Groups whose steps need no global reduction: every instance CH or K93 and not
+// conserved (one instance in one segment, or a tile map for the rest); batches
+// only in mode 2, since every sleeping tile still takes a workgroup.
 bool single_launch_ok(const dba_ctx* ctx, const Group* g) {
-    if (!ctx->single_launch || g->ids.size() != 1 || g->nseg != 1) return false;
-    return single_launch_params(ctx->inst.at(g->ids[0])->p, (int)g->total_tiles);
+    if (ctx->single_launch == 0 || (ctx->single_launch == 1 && g->ids.size() != 1)) return false;
+    if (!(g->ids.size() == 1 && g->nseg == 1) && !g->d_dmap) return false;
+    for (int id : g->ids)
+        if (!single_launch_params(ctx->inst.at(id)->p)) return false;
+    return true;
 }
 
 template <int TB, int NT>
@@ -529,9 +550,9 @@ void launch_dense_tb(const Group* g, hipStream_t s, int first, int step) {
     const dim3 grid((unsigned)g->total_tiles);
     if (g->model == DBA_MODEL_CH) {
         constexpr int NTC = NT <= TB * TB ? NT : TB * TB;
-        hipLaunchKernelGGL((ch_dense_kernel<TB, NTC>), grid, dim3(NTC), 0, s, g->d_insts, first, step);
+        hipLaunchKernelGGL((ch_dense_kernel<TB, NTC>), grid, dim3(NTC), 0, s, g->d_insts, g->d_dmap, first, step);
     } else {
-        hipLaunchKernelGGL((k93_dense_kernel<TB, NT>), grid, dim3(NT), 0, s, g->d_insts, first, step);
+        hipLaunchKernelGGL((k93_dense_kernel<TB, NT>), grid, dim3(NT), 0, s, g->d_insts, g->d_dmap, first, step);
     }
 }
 
@@ -612,10 +633,17 @@ dba_status enqueue_steps(dba_ctx* ctx, const std::vector<Group*>& gs, int n_step
                 b = pooled_event(ctx);
                 (void)hipEventRecord(a, ctx->stream);
             }
-            if (s < n_steps)
+            if (s < n_steps) {
                 launch_dense(g, ctx->stream, s == 0, s);
-            else
+            } else if (!g->d_dmap) {
                 hipLaunchKernelGGL(dense_canon_kernel, dim3(1), dim3(kFinNT), 0, ctx->stream, g->d_insts, n_steps);
+            } else {
+                hipLaunchKernelGGL(dense_canon_seg_kernel, dim3((unsigned)g->nseg), dim3(kFinNT), 0, ctx->stream,
+                                   g->d_insts, g->d_segs, g->d_parts, g->d_counts, n_steps);
+                LAUNCH_CHECK();
+                hipLaunchKernelGGL(dense_canon_merge_kernel, dim3(1), dim3(kFinNT), 0, ctx->stream, g->d_insts,
+                                   g->d_isegs, (int)g->ids.size(), g->d_parts, g->d_counts, n_steps);
+            }
             LAUNCH_CHECK();
             if (timed) {
                 (void)hipEventRecord(b, ctx->stream);
@@ -911,7 +939,7 @@ dba_status dba_instance_info(dba_ctx* ctx, int32_t id, dba_info* out) {
     out->last_dt = scal[3];
     out->cons_offset = scal[8];
     out->footprint_bytes = in->footprint;
-    out->single_launch = ctx->single_launch && single_launch_params(in->p, in->h.nt) ? 1 : 0;
+    out->single_launch = ctx->single_launch != 0 && single_launch_params(in->p) ? 1 : 0;
     out->reserved = 0;
     return DBA_OK;
 }
@@ -995,9 +1023,10 @@ dba_status dba_step_graph(dba_ctx* ctx, const int32_t* ids, int32_t n_ids, int32
     return DBA_OK;
 }
 
-dba_status dba_set_single_launch(dba_ctx* ctx, int32_t enable) {
+dba_status dba_set_single_launch(dba_ctx* ctx, int32_t mode) {
     if (!ctx) return fail(DBA_ERR_INVALID, "null ctx");
-    const bool on = enable != 0;
+    if (mode < 0 || mode > 2) return fail(DBA_ERR_INVALID, "single-launch mode must be 0, 1 or 2 (got %d)", mode);
+    const int on = mode;
     if (on != ctx->single_launch) {
         // captured graphs hold the other launch sequence
         HIP_TRY(hipStreamSynchronize(ctx->stream));

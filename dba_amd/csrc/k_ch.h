@@ -208,16 +208,18 @@ __global__ __launch_bounds__(NT) void ch_step_kernel(const WorkList wl) {
     }
 }
 
-// Single-launch DBA step of a single-instance CH group: one workgroup per
-// tile and no worklist or finalize launch. Workgroup t derives the awake bit and
+// Single-launch DBA step of a CH group: one workgroup per tile of every
+// instance and no worklist or finalize launch. Workgroup t derives the awake bit and
 // buffer parity of its 3x3 owner tiles itself: at the first step of a run from
 // the bitmaps, afterwards from the per-tile bytes the previous step wrote
 // (awake = the dilation of the detector bits, exactly the finalize's rule). A
 // sleeping tile writes its bytes and exits; an awake one runs ch_tile as in the
-// worklist kernel. dense_canon_kernel restores the bitmaps and counters at the
-// end of the run. Results are bit-identical to the worklist path.
+// worklist kernel. The canonicalising kernels (dense_canon_*) restore the
+// bitmaps and counters at the end of the run. Results are bit-identical to the
+// worklist path.
 template <int TB, int NT>
-__global__ __launch_bounds__(NT) void ch_dense_kernel(const Inst* __restrict__ insts, int first, int s) {
+__global__ __launch_bounds__(NT) void ch_dense_kernel(const Inst* __restrict__ insts,
+                                                      const int32_t* __restrict__ dmap, int first, int s) {
     constexpr int W2 = TB + 4;
     constexpr int W1 = TB + 2;
     __shared__ double sc[W2 * W2];
@@ -225,10 +227,8 @@ __global__ __launch_bounds__(NT) void ch_dense_kernel(const Inst* __restrict__ i
     __shared__ double sred[NT / 64 + 1];
     __shared__ int sflag[NT / 64 + 1];
     __shared__ NbTable nb;
-    CInst& I = ((CInst*)insts)[0];
-    // the worklist kernels' XCD deal: contiguous tile ranges per XCD, so halo
-    // rows written by a neighbour tile last step are in this XCD's L2
-    const int t = xcd_deal(blockIdx.x, gridDim.x, 0, gridDim.x);
+    int t;
+    CInst& I = dense_item(insts, dmap, &t);
     const int prev = s & 1, next = prev ^ 1;
     const bool regular = I.mode == 0;
     dense_nb_fill(nb, I, t, first, prev, regular, 1);

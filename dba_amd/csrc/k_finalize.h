@@ -424,6 +424,23 @@ __global__ __launch_bounds__(kFinNT) void finalize_merge_kernel(const Inst* __re
     finalize_merge_body<kFinNT>(insts, nseg, isegs, n_inst, parts, counts, offsets, rotate, si);
 }
 
+// Instance and tile of a single-launch step's workgroup. Workgroups are dealt
+// to XCDs in contiguous ranges (the worklist kernels' deal), so halo rows a
+// neighbour tile wrote last step are in this XCD's L2. A group of several
+// instances or segments maps the dealt index through dmap (instance << 24 |
+// tile, the instances' tiles in group order); one instance in one segment
+// uses the index itself.
+__device__ __forceinline__ CInst& dense_item(const Inst* insts, const int32_t* dmap, int* t) {
+    const int b = xcd_deal(blockIdx.x, gridDim.x, 0, gridDim.x);
+    if (!dmap) {
+        *t = b;
+        return ((CInst*)insts)[0];
+    }
+    const uint32_t d = (uint32_t)dmap[b];
+    *t = (int)(d & 0xFFFFFFu);
+    return ((CInst*)insts)[d >> 24];
+}
+
 // Neighbour table of a single-launch step (threads 0..8; caller synchronises):
 // per set, buffer parity and awake bit of each 3x3 owner tile, from the bitmaps
 // at the first step of a run, afterwards from the previous step's per-tile bytes
@@ -453,24 +470,24 @@ __device__ __forceinline__ void dense_nb_fill(NbTable& nb, CInst& I, int t, int 
     for (int f = 0; f < kMaxFields; f++) nb.shift[f][d] = 0.0;
 }
 
-// End of a run of n >= 1 single-launch steps of a single-instance 2D group (at
-// most one segment): the state the worklist path and the host read,
-// from the per-tile bytes of the last step. Awake bitmap = the dilation of the
-// last detector bits, parity bitmap, detector bytes (a tile that did not fire
-// holds 0, the invariant the finalize's dilation relies on), steps, cell
-// updates, the awake count, and the simulated time advanced step by step as the
-// finalize advances it. One thread per tile.
-__global__ __launch_bounds__(kFinNT) void dense_canon_kernel(const Inst* __restrict__ insts, int n) {
-    __shared__ unsigned long long scells[kFinNT / 64];
-    __shared__ int scnt[kFinNT / 64];
-    CInst& I = ((CInst*)insts)[0];
-    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+// End of a run of n >= 1 single-launch steps: the state the worklist path and
+// the host read, from the per-tile bytes of the last step. Awake bitmap = the
+// dilation of the last detector bits, parity bitmap, detector bytes (a tile
+// that did not fire holds 0, the invariant the finalize's dilation relies on),
+// steps, cell updates, the awake count, and the simulated time advanced step by
+// step as the finalize advances it. One thread per tile of a segment (the
+// segment starts on a 64-tile word); returns, in every thread, the segment's
+// cell updates and awake-tile count.
+__device__ __forceinline__ void dense_canon_seg(CInst& I, int t0, int ntile, int n, unsigned long long* scells,
+                                                int* scnt, unsigned long long* cells_out, int* cnt_out) {
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int nt = I.nt, ntx = I.ntx;
-    const bool valid = t < nt;
+    const int t = t0 + tid;
+    const bool valid = tid < ntile;
     const bool regular = I.mode == 0;
     const int last = n & 1;
     const int nsets = I.nsets;
-    const int nw = (nt + 63) >> 6;
+    const int nw = (ntile + 63) >> 6;
     const int r = valid ? I.druns[t] : 0;
     uint64_t wa = 0ull;
     for (int s = 0; s < nsets; s++) {
@@ -486,8 +503,8 @@ __global__ __launch_bounds__(kFinNT) void dense_canon_kernel(const Inst* __restr
         const uint64_t ws = __ballot(a), wp = __ballot(p);
         wa |= ws;
         if (lane == 0 && wid < nw) {
-            I.par[s][wid] = wp;
-            if (!regular) I.a_cur[s][wid] = ws;
+            I.par[s][(t0 >> 6) + wid] = wp;
+            if (!regular) I.a_cur[s][(t0 >> 6) + wid] = ws;
         }
     }
     if (valid) I.druns[t] = 0;
@@ -499,21 +516,78 @@ __global__ __launch_bounds__(kFinNT) void dense_canon_kernel(const Inst* __restr
         scnt[wid] = __popcll(wa);
     }
     __syncthreads();
-    if (t == 0) {
+    unsigned long long c = 0;
+    int cnt = 0;
+    for (int w = 0; w < nw; w++) {
+        c += scells[w];
+        cnt += scnt[w];
+    }
+    *cells_out = c;
+    *cnt_out = cnt;
+}
+
+// The instance counters after n single-launch steps (one thread).
+__device__ __forceinline__ void dense_canon_finish(CInst& I, int n, unsigned long long cells, int64_t cnt) {
+    I.ctr[0] = I.ctr[0] + n;
+    I.ctr[1] = I.ctr[1] + (int64_t)cells;
+    I.ctr[2] = cnt;
+    const double dt = I.scal[1];
+    double tm = I.scal[2];
+    for (int i = 0; i < n; i++) tm += dt;
+    I.scal[2] = tm;
+    I.scal[3] = dt;
+}
+
+// One instance in one segment: one workgroup does it all.
+__global__ __launch_bounds__(kFinNT) void dense_canon_kernel(const Inst* __restrict__ insts, int n) {
+    __shared__ unsigned long long scells[kFinNT / 64];
+    __shared__ int scnt[kFinNT / 64];
+    CInst& I = ((CInst*)insts)[0];
+    unsigned long long c;
+    int cnt;
+    dense_canon_seg(I, 0, I.nt, n, scells, scnt, &c, &cnt);
+    if (threadIdx.x == 0) dense_canon_finish(I, n, c, cnt);
+}
+
+// Several segments or instances: one workgroup per segment writes its partial
+// sums (parts[seg].cells, counts[seg]) ...
+__global__ __launch_bounds__(kFinNT) void dense_canon_seg_kernel(const Inst* __restrict__ insts,
+                                                                 const Seg* __restrict__ segs,
+                                                                 SegPart* __restrict__ parts,
+                                                                 int32_t* __restrict__ counts, int n) {
+    __shared__ unsigned long long scells[kFinNT / 64];
+    __shared__ int scnt[kFinNT / 64];
+    const Seg sg = segs[blockIdx.x];
+    CInst& I = ((CInst*)insts)[sg.inst];
+    unsigned long long c;
+    int cnt;
+    dense_canon_seg(I, sg.tile0, sg.ntile, n, scells, scnt, &c, &cnt);
+    if (threadIdx.x == 0) {
+        parts[blockIdx.x].cells = c;
+        counts[blockIdx.x] = cnt;
+    }
+}
+
+// ... and one wave per instance sums them in segment order.
+__global__ __launch_bounds__(kFinNT) void dense_canon_merge_kernel(const Inst* __restrict__ insts,
+                                                                   const InstSegs* __restrict__ isegs, int n_inst,
+                                                                   const SegPart* __restrict__ parts,
+                                                                   const int32_t* __restrict__ counts, int n) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int i = wid; i < n_inst; i += kFinNT / 64) {
+        const InstSegs r = isegs[i];
         unsigned long long c = 0;
-        int cnt = 0;
-        for (int w = 0; w < nw; w++) {
-            c += scells[w];
-            cnt += scnt[w];
+        int64_t cnt = 0;
+        for (int q = r.first + lane; q < r.first + r.count; q += 64) {
+            c += parts[q].cells;
+            cnt += counts[q];
         }
-        I.ctr[0] = I.ctr[0] + n;
-        I.ctr[1] = I.ctr[1] + (int64_t)c;
-        I.ctr[2] = cnt;
-        const double dt = I.scal[1];
-        double tm = I.scal[2];
-        for (int i = 0; i < n; i++) tm += dt;
-        I.scal[2] = tm;
-        I.scal[3] = dt;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            c += __shfl_down(c, off, 64);
+            cnt += __shfl_down(cnt, off, 64);
+        }
+        if (lane == 0) dense_canon_finish(((CInst*)insts)[i], n, c, cnt);
     }
 }
 

@@ -181,17 +181,18 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k9
     }
 }
 
-// Single-launch step of a single-instance K93 group (see ch_dense_kernel): per
+// Single-launch step of a K93 group (see ch_dense_kernel): per
 // set, awake bits and parities of the 3x3 owner tiles from the bitmaps (first
 // step) or the previous step's bytes; the tile runs when either set is awake.
 template <int TB, int NT>
-__global__ __launch_bounds__(NT) void k93_dense_kernel(const Inst* __restrict__ insts, int first, int s) {
+__global__ __launch_bounds__(NT) void k93_dense_kernel(const Inst* __restrict__ insts,
+                                                       const int32_t* __restrict__ dmap, int first, int s) {
     __shared__ K93Lds<TB> L;
     __shared__ int sflag[NT / 64 + 1];
     __shared__ NbTable nb;
-    CInst& I = ((CInst*)insts)[0];
+    int t;
+    CInst& I = dense_item(insts, dmap, &t);
     const int nt = I.nt;
-    const int t = xcd_deal(blockIdx.x, gridDim.x, 0, gridDim.x);
     const int prev = s & 1, next = prev ^ 1;
     const bool regular = I.mode == 0;
     dense_nb_fill(nb, I, t, first, prev, regular, 2);

@@ -152,10 +152,11 @@ class Context:
     def sync(self):
         abi.check(self.lib.dba_sync(self.h))
 
-    def set_single_launch(self, enable):
-        """Single-launch steps for eligible groups (on by default); off forces the
-        worklist path (step kernel + finalize). Results are identical."""
-        abi.check(self.lib.dba_set_single_launch(self.h, 1 if enable else 0))
+    def set_single_launch(self, mode):
+        """Single-launch steps: 0 (or False) forces the worklist path (step kernel
+        + finalize); 1 (or True, the default) uses single-launch steps for groups of
+        one instance; 2 also for batches. Results are identical in every mode."""
+        abi.check(self.lib.dba_set_single_launch(self.h, int(mode)))
 
     def timing(self, enable):
         abi.check(self.lib.dba_timing_enable(self.h, 1 if enable else 0))

@@ -75,9 +75,9 @@ def test_threads_do_not_change_results(ctx, tile):
 def test_autotune_picks_a_timed_candidate(ctx):
     items = small_sweep(4)
     best, table = batch.autotune(ctx, items, trial_steps=20, warm_steps=5, chunk=10)
-    assert len(table) == 9
-    assert best in [(t, n) for t, n, _, _ in table]
-    assert all(ms > 0 for _, _, ms, _ in table)
+    assert len(table) == 18
+    assert best in [(t, n, m) for t, n, m, _, _ in table]
+    assert all(ms > 0 for _, _, _, ms, _ in table)
 
 
 def run_alone(ctx, r, tiles):

@@ -1,6 +1,7 @@
 """GPU: single-launch steps (one launch per step, awake sets derived in every
-tile's workgroup, one canonicalising kernel per call or graph chunk) against the
-worklist path (step kernel + finalize) on the same instance: fields, awake sets,
+tile's workgroup, canonicalising kernels per call or graph chunk) against the
+worklist path (step kernel + finalize) on the same instances, alone and in
+batches: fields, awake sets,
 step and cell-update counters, history, simulated time and the rebuilt worklist
 are identical. The worklist path is pinned to the oracle by tests/test_gpu_ch.py
 (which itself runs through the single-launch path for eligible instances)."""
@@ -103,9 +104,115 @@ def test_k93_single_launch_equals_worklist(ctx, tile, mode):
         assert not np.array_equal(p_awake, t_awake)   # the two sets differ
 
 
-def test_conserved_and_batched_use_the_worklist_path(ctx):
-    """Conserved DBA and multi-instance groups are not single-launch eligible;
-    they run (and match the oracle elsewhere) with the switch on."""
+@pytest.mark.parametrize("n,tile,stencil", [(1024, 16, 9), (2048, 32, 5)])
+def test_multi_segment_single_launch_equals_worklist(ctx, n, tile, stencil):
+    """4096 tiles, four 1024-tile segments: single-launch steps through the tile
+    map, one canonicalising workgroup per segment and the per-instance merge."""
+    p, c = ch_case(n, tile, abi.MODE_DBA, stencil, "periodic")
+    plan = [(1, 0), (6, 0), (40, 20), (3, 0)]
+    a = run(ctx, p, c, False, plan)
+    b = run(ctx, p, c, True, plan)
+    assert np.array_equal(a["c"], b["c"])
+    for x, y in zip(a["masks"], b["masks"]):
+        assert np.array_equal(x, y)
+    assert a["steps"] == b["steps"] and a["updates"] == b["updates"] and a["time"] == b["time"]
+    assert np.array_equal(a["hist"], b["hist"])
+    assert np.array_equal(a["work"], b["work"])
+    assert a["masks"][-1].sum() < a["masks"][-1].size
+
+
+def batch_cases():
+    """Uneven K93 dendrites (one over 1024 tiles) and CH grids in DBA and regular mode."""
+    out = []
+    for n, seeds, mode in ((128, [(40, 60)], abi.MODE_DBA), (544, [(100, 300), (400, 120)], abi.MODE_DBA),
+                           (96, [(48, 48)], abi.MODE_REGULAR)):
+        p = configs.dendrite_k93_params(n=n, tile=16, mode=mode, history_cap=200)
+        phi, T = configs.dendrite_k93_ic(n, n, seeds=seeds)
+        out.append(("k93", p, [phi, T]))
+    for n, mode in ((256, abi.MODE_DBA), (160, abi.MODE_REGULAR)):
+        p, c = ch_case(n, 16, mode, 9, "periodic")
+        p.history_cap = 200
+        out.append(("ch", p, [c]))
+    return out
+
+
+def batch_run(ctx, cases, single, plan):
+    ctx.set_single_launch(2 if single else 0)
+    insts = []
+    for _, p, fields in cases:
+        inst = ctx.create(p)
+        for f, a in enumerate(fields):
+            inst.upload(f, a)
+        insts.append(inst)
+    groups = [[i for i, (m, _, _) in zip(insts, cases) if m == kind] for kind in ("k93", "ch")]
+    groups = [g for g in groups if g]
+    masks = []
+    for steps, chunk in plan:
+        for g in groups:
+            ctx.step(g, steps, chunk)
+        masks.append([[inst.active(s)[1] for s in range(2 if kind == "k93" else 1)]
+                      for inst, (kind, _, _) in zip(insts, cases)])
+    out = []
+    for inst, (_, _, fields) in zip(insts, cases):
+        info = inst.info()
+        out.append(dict(f=[inst.download(f) for f in range(len(fields))], steps=info.steps_done,
+                        updates=info.cell_updates, time=info.sim_time, hist=inst.history(),
+                        work=inst.worklist(), single=info.single_launch))
+        inst.destroy()
+    ctx.set_single_launch(True)
+    return out, masks
+
+
+def test_batched_single_launch_equals_worklist(ctx):
+    """Batches of uneven instances (mixed DBA / regular modes, one instance over
+    1024 tiles) in single-launch steps through the tile map give the worklist
+    path's fields, awake sets, counters, times, history and rebuilt worklists."""
+    cases = batch_cases()
+    plan = [(1, 0), (5, 0), (60, 20), (2, 0)]
+    a, ma = batch_run(ctx, cases, False, plan)
+    b, mb = batch_run(ctx, cases, True, plan)
+    for x, y in zip(a, b):
+        assert x["single"] == 0 and y["single"] == 1
+        for u, v in zip(x["f"], y["f"]):
+            assert np.array_equal(u, v)
+        assert x["steps"] == y["steps"] and x["updates"] == y["updates"] and x["time"] == y["time"]
+        assert np.array_equal(x["hist"], y["hist"])
+        assert np.array_equal(x["work"], y["work"])
+    for sa, sb in zip(ma, mb):
+        for ia, ib in zip(sa, sb):
+            for u, v in zip(ia, ib):
+                assert np.array_equal(u, v)
+    # the large dendrite has sleeping tiles at the end
+    p_awake = ma[-1][1][0]
+    assert 0 < p_awake.sum() < p_awake.size
+
+
+def test_single_launch_modes(ctx):
+    """Mode 1 leaves batches on the worklist path, mode 2 takes them too, other
+    modes are rejected; the steps agree in every mode."""
+    cases = batch_cases()[3:]
+    plan = [(3, 0), (20, 10)]
+    ref, mref = batch_run(ctx, cases, False, plan)
+    ctx.set_single_launch(1)
+    insts = []
+    for _, p, fields in cases:
+        inst = ctx.create(p)
+        inst.upload(0, fields[0])
+        insts.append(inst)
+    for steps, chunk in plan:
+        ctx.step(insts, steps, chunk)
+    for inst, r in zip(insts, ref):
+        assert np.array_equal(inst.download(0), r["f"][0])
+        inst.destroy()
+    for bad in (-1, 3):
+        with pytest.raises(abi.DbaError):
+            ctx.set_single_launch(bad)
+    ctx.set_single_launch(True)
+
+
+def test_conserved_uses_the_worklist_path(ctx):
+    """Conserved DBA is not single-launch eligible (its steps need a global sum);
+    it runs (and matches the oracle elsewhere) with the switch on."""
     n = 64
     p = configs.ch_params(n=n, tile=16, mode=abi.MODE_CONSERVED, eps=1e-3)
     c = configs.ch_ic(n, n)
