@@ -326,7 +326,10 @@ def bench_serve(args):
     with Context(0) as ctx:
         free, total = ctx.device_memory()
         be = serve.LibraryBackend(ctx)
-        srv = serve.Server(be, budget, chunk=args.chunk, tiles=tiles, target_occupancy=0.8, gain=2.0, log=log)
+        # fully active CH batches take one launch per step; sparse dendrite batches the merged worklist
+        modes = {"ch": 2, "dendrite": 1} if args.serve_modes == "split" else None
+        srv = serve.Server(be, budget, chunk=args.chunk, tiles=tiles, target_occupancy=0.8, gain=2.0, log=log,
+                           modes=modes)
         done = srv.run(rs, rate=args.rate)
         rep = serve.report(done, srv.wall, srv.occupancy)
         log(f"served {rep['completed']} in {srv.wall:.1f} s, p50 {rep['p50_latency_s']:.3f} s, "
@@ -365,7 +368,8 @@ def bench_serve(args):
         "config": {"workload": f"serve: {args.requests} requests, 50% Kobayashi-93 dendrite / 50% reactive CH, "
                                "sides log-U[256,4096], steps log-U[1e3,5e4]",
                    "budget_bytes": budget, "device_total_bytes": total, "graph_chunk": args.chunk,
-                   "initial_rate_per_s": args.rate, "target_occupancy": 0.8},
+                   "initial_rate_per_s": args.rate, "target_occupancy": 0.8,
+                   "single_launch_modes": modes or "library default, one call"},
         **rep,
         "latency_s": {"p50": rep["p50_latency_s"], "p90": serve.percentile(lat, 90), "p99": rep["p99_latency_s"],
                       "max": lat[-1]},
@@ -395,6 +399,8 @@ def main():
     ap.add_argument("--tile", type=int, default=None, help="activation tile edge override")
     ap.add_argument("--threads", type=int, default=0, help="workgroup size override (64/128/256)")
     ap.add_argument("--tune-steps", type=int, default=200, help="batch: timed steps per autotuner candidate")
+    ap.add_argument("--serve-modes", default="split", choices=("split", "one"),
+                    help="serve: step CH (single-launch mode 2) and dendrites (mode 1) apart, or all in one call")
     ap.add_argument("--single-launch", type=int, default=1, choices=(0, 1),
                     help="one-instance workloads: 0 forces the worklist path (step kernel + finalize)")
     ap.add_argument("--requests", type=int, default=300, help="serve: requests in the stream")

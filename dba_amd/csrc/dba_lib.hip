@@ -109,7 +109,12 @@ struct Group {
 struct GraphKey {
     std::vector<int> ids;
     int chunk;
-    bool operator<(const GraphKey& o) const { return chunk != o.chunk ? chunk < o.chunk : ids < o.ids; }
+    int single_launch;  // the mode the graph was captured in
+    bool operator<(const GraphKey& o) const {
+        if (chunk != o.chunk) return chunk < o.chunk;
+        if (single_launch != o.single_launch) return single_launch < o.single_launch;
+        return ids < o.ids;
+    }
 };
 
 }  // namespace
@@ -999,7 +1004,7 @@ dba_status dba_step_graph(dba_ctx* ctx, const int32_t* ids, int32_t n_ids, int32
     if (st != DBA_OK) return st;
     int reps = n_steps / chunk, rem = n_steps % chunk;
     if (reps > 0) {
-        GraphKey key{std::vector<int>(ids, ids + n_ids), chunk};
+        GraphKey key{std::vector<int>(ids, ids + n_ids), chunk, ctx->single_launch};
         auto it = ctx->graphs.find(key);
         hipGraphExec_t exec = nullptr;
         if (it == ctx->graphs.end()) {
@@ -1026,14 +1031,8 @@ dba_status dba_step_graph(dba_ctx* ctx, const int32_t* ids, int32_t n_ids, int32
 dba_status dba_set_single_launch(dba_ctx* ctx, int32_t mode) {
     if (!ctx) return fail(DBA_ERR_INVALID, "null ctx");
     if (mode < 0 || mode > 2) return fail(DBA_ERR_INVALID, "single-launch mode must be 0, 1 or 2 (got %d)", mode);
-    const int on = mode;
-    if (on != ctx->single_launch) {
-        // captured graphs hold the other launch sequence
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
-        for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second.exec);
-        ctx->graphs.clear();
-        ctx->single_launch = on;
-    }
+    // captured graphs are cached per mode, so switching keeps them
+    ctx->single_launch = mode;
     return DBA_OK;
 }
 

@@ -135,8 +135,12 @@ class LibraryBackend:
             inst.upload(f, a)
         return inst
 
-    def step(self, handles, n, chunk):
+    def step(self, handles, n, chunk, mode=None):
+        if mode is not None:
+            self.ctx.set_single_launch(mode)
         self.ctx.step(handles, n, graph_chunk=chunk)
+        if mode is not None:
+            self.ctx.set_single_launch(1)
 
     def sync(self):
         self.ctx.sync()
@@ -155,8 +159,11 @@ class Server:
     """Continuous-batching loop over a backend (LibraryBackend on the GPU)."""
 
     def __init__(self, backend, capacity_bytes, chunk=100, tiles=None, threads=None, keep_fields=False,
-                 target_occupancy=None, gain=2.0, log=None):
+                 target_occupancy=None, gain=2.0, log=None, modes=None):
         self.b = backend
+        # single-launch mode per request kind (dba_set_single_launch), or None:
+        # all kinds in one call in the library's default mode
+        self.modes = modes
         self.capacity = int(capacity_bytes)
         self.chunk = chunk
         self.tiles = tiles or {"dendrite": 16, "ch": 32}
@@ -237,7 +244,12 @@ class Server:
             if resident:
                 # one tick for the resident set, asynchronous on the device
                 k = min(self.chunk, min(v[1] for v in resident.values()))
-                self.b.step([v[0] for v in resident.values()], k, self.chunk)
+                if self.modes is None:
+                    self.b.step([v[0] for v in resident.values()], k, self.chunk)
+                else:
+                    for kind in sorted({v[2].kind for v in resident.values()}):
+                        hs = [v[0] for v in resident.values() if v[2].kind == kind]
+                        self.b.step(hs, k, self.chunk, self.modes.get(kind, 1))
             # admissions overlap the tick: new instances are set up on another stream
             incoming = admit()
             if k:

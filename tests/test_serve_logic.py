@@ -28,6 +28,7 @@ class FakeBackend:
         self.bpc = bytes_per_cell
         self.steps = {}
         self.calls = []
+        self.modes = []
         self.live = set()
         self.next = 0
 
@@ -42,10 +43,11 @@ class FakeBackend:
         self.live.add(h)
         return h
 
-    def step(self, handles, n, chunk):
+    def step(self, handles, n, chunk, mode=None):
         assert n >= 1 and len(set(handles)) == len(handles)
         assert all(h in self.live for h in handles)
         self.calls.append((tuple(handles), n))
+        self.modes.append(mode)
         for h in handles:
             self.steps[h] += n
         self.tm.t += 1e-9 * n * sum(h[1] for h in handles)
@@ -155,6 +157,23 @@ def test_continuous_batching_overlaps_requests():
     srv = serve.Server(be, 10 ** 12, chunk=50)
     srv.run(rs, clock=tm.clock, sleep=tm.sleep)
     assert max(len(h) for h, _ in be.calls) == 12
+
+
+def test_modes_step_each_kind_in_its_own_call():
+    """With per-kind single-launch modes each tick steps the dendrites and the CH
+    requests in separate calls, in their modes; every request still runs exactly
+    its steps."""
+    tm = FakeTime()
+    be = FakeBackend(tm)
+    rs = small_requests(16)
+    for r in rs:
+        r.arrival = 0.0
+    srv = serve.Server(be, 10 ** 12, chunk=30, modes={"ch": 2, "dendrite": 1})
+    done = srv.run(rs, clock=tm.clock, sleep=tm.sleep)
+    by = {r.rid: r for r in rs}
+    assert all(c.steps == by[c.rid].steps for c in done) and len(done) == len(rs)
+    assert set(be.modes) == {1, 2}
+    assert max(len(h) for h, _ in be.calls) < len(rs)   # never both kinds in one call
 
 
 def test_request_larger_than_budget_is_rejected():
