@@ -101,11 +101,19 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
     constexpr int W1 = TB + 2;
     constexpr int A2 = W2 * W2;
     constexpr int A1 = W1 * W1;
-    static_assert(4 * A2 >= 4 * (TB + 1) * TB && 4 * A1 >= 4 * (TB + 1) * TB, "flux reuse");
-    __shared__ double sP[4 * A2];    // primitives r, vx, vy, p; later x-face fluxes
+    constexpr int FW = TB + 1;       // flux array row width (x faces) / rows (y faces)
+    constexpr int NF = FW * TB;      // faces per direction
+    constexpr int YS = (NF + NT - 1) / NT;
+    static_assert(A2 >= NF, "y-face fluxes reuse the primitive space");
+    // The gradients are not stored: the flux phases recompute them from the
+    // primitives with the expression phase 2 uses (the same doubles), except the
+    // stale ones a sleeping right / top neighbour kept. 33 KB at TB = 16 (4
+    // workgroups per CU instead of 3 with stored gradients).
+    __shared__ double sP[4 * A2];    // primitives r, vx, vy, p; later y-face fluxes
     __shared__ double sPR[4 * A1];   // primes
-    __shared__ double sGX[4 * A1];   // x-gradients; later y-face fluxes
-    __shared__ double sGY[4 * A1];   // y-gradients
+    __shared__ double sXF[4 * NF];   // x-face fluxes
+    __shared__ double sGXR[4][TB];   // x-gradients of a sleeping right neighbour's left column
+    __shared__ double sGYT[4][TB];   // y-gradients of a sleeping top neighbour's bottom row
     __shared__ double sred[NT / 64 + 1];
     __shared__ int sflag[NT / 64 + 1];
     __shared__ NbTable nb;
@@ -208,13 +216,13 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
                         const int row = map_index(y0 + ly - 1, ny, wall ? 1 : 0) & (TB - 1);
                         for (int q = 0; q < 4; q++) {
                             sPR[q * A1 + k] = e[E::LEFT + q * TB + row];
-                            sGX[q * A1 + k] = e[E::LEFT + (4 + q) * TB + row];
+                            sGXR[q][ly - 1] = e[E::LEFT + (4 + q) * TB + row];
                         }
                     } else {
                         const int col = map_index(x0 + lx - 1, nx, 0) & (TB - 1);
                         for (int q = 0; q < 4; q++) {
                             sPR[q * A1 + k] = e[E::BOTTOM + q * TB + col];
-                            sGY[q * A1 + k] = e[E::BOTTOM + (4 + q) * TB + col];
+                            sGYT[q][lx - 1] = e[E::BOTTOM + (4 + q) * TB + col];
                         }
                     }
                 }
@@ -241,14 +249,6 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
             sPR[A1 + k] = up;
             sPR[2 * A1 + k] = vp;
             sPR[3 * A1 + k] = pp;
-            sGX[k] = r_x;
-            sGX[A1 + k] = u_x;
-            sGX[2 * A1 + k] = v_x;
-            sGX[3 * A1 + k] = p_x;
-            sGY[k] = r_y;
-            sGY[A1 + k] = u_y;
-            sGY[2 * A1 + k] = v_y;
-            sGY[3 * A1 + k] = p_y;
             if (d == 4) {
                 const double metric = I.metric_kind == 1
                     ? fabs(r_x) + fabs(r_y)
@@ -276,9 +276,19 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
         }
         __syncthreads();
 
-        // 3a. x-face fluxes of W1 columns 0..TB on the tile rows (into the primitive space)
-        constexpr int FW = TB + 1;  // flux array row width (x faces) / rows (y faces)
-        for (int k = tid; k < FW * TB; k += NT) {
+        // Grad_x / Grad_y of W1 cell (lx, ly) recomputed from the primitives
+        // (example_Kelvin_Helmholtz/KH_DBA.cu:249-275, as in phase 2)
+        auto grad_x = [&](int q, int lx, int ly) {
+            const int p0 = (lx + 1) + (ly + 1) * W2;
+            return (sP[q * A2 + p0 + 1] - sP[q * A2 + p0 - 1]) / 2.0 / c.dx;
+        };
+        auto grad_y = [&](int q, int lx, int ly) {
+            const int p0 = (lx + 1) + (ly + 1) * W2;
+            return (sP[q * A2 + p0 + W2] - sP[q * A2 + p0 - W2]) / 2.0 / c.dy;
+        };
+
+        // 3a. x-face fluxes of W1 columns 0..TB on the tile rows
+        for (int k = tid; k < NF; k += NT) {
             const int lx = k % FW, ly = 1 + k / FW;  // W1 coordinates of the cell left of the face
             const int a = lx + ly * W1, b = a + 1;
             double fm, fe, fu, fv;
@@ -291,10 +301,13 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
                 fv = e[2 * TB];
                 fe = e[3 * TB];
             } else {
+                // the right cell is a sleeping right neighbour's: its stored gradients
+                const bool bstale = lx == TB && !nb.act[0][nb_dir1<TB, 1>(TB + 1) + 3 * nb_dir1<TB, 1>(ly)];
                 double L[4], R[4];
                 for (int q = 0; q < 4; q++) {
-                    L[q] = sPR[q * A1 + b] - sGX[q * A1 + b] * c.dx / 2.0;  // face[0] of the right cell
-                    R[q] = sPR[q * A1 + a] + sGX[q * A1 + a] * c.dx / 2.0;  // face[1] of the left cell
+                    const double gb = bstale ? sGXR[q][ly - 1] : grad_x(q, lx + 1, ly);
+                    L[q] = sPR[q * A1 + b] - gb * c.dx / 2.0;                    // face[0] of the right cell
+                    R[q] = sPR[q * A1 + a] + grad_x(q, lx, ly) * c.dx / 2.0;  // face[1] of the left cell
                 }
                 kh_rusanov(c, L[0], R[0], L[3], R[3], L[1], R[1], L[2], R[2], &fm, &fe, &fu, &fv);
                 if (dba && lx == TB) {
@@ -305,15 +318,18 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
                     e[3 * TB] = fe;
                 }
             }
-            sP[0 * A2 + k] = fm;
-            sP[1 * A2 + k] = fu;
-            sP[2 * A2 + k] = fv;
-            sP[3 * A2 + k] = fe;
+            sXF[0 * NF + k] = fm;
+            sXF[1 * NF + k] = fu;
+            sXF[2 * NF + k] = fv;
+            sXF[3 * NF + k] = fe;
         }
-        // 3b. y-face fluxes of W1 rows 0..TB on the tile columns need sGX no longer
-        //     only after 3a has read it
-        __syncthreads();
-        for (int k = tid; k < FW * TB; k += NT) {
+        // 3b. y-face fluxes of W1 rows 0..TB on the tile columns, held in registers
+        //     until every thread has read the primitives, then stored over them
+        double yf[YS][4];
+#pragma unroll
+        for (int i = 0; i < YS; i++) {
+            const int k = tid + i * NT;
+            if (k >= NF) break;
             const int ly = k / TB, lx = 1 + k % TB;  // W1 coordinates of the cell below the face
             const int a = lx + ly * W1, b = a + W1;
             double fm, fe, fu, fv;
@@ -326,10 +342,12 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
                 fv = e[2 * TB];
                 fe = e[3 * TB];
             } else {
+                const bool bstale = ly == TB && !nb.act[0][nb_dir1<TB, 1>(lx) + 3 * nb_dir1<TB, 1>(TB + 1)];
                 double L[4], R[4];
                 for (int q = 0; q < 4; q++) {
-                    L[q] = sPR[q * A1 + b] - sGY[q * A1 + b] * c.dy / 2.0;  // face[2] of the upper cell
-                    R[q] = sPR[q * A1 + a] + sGY[q * A1 + a] * c.dy / 2.0;  // face[3] of the lower cell
+                    const double gb = bstale ? sGYT[q][lx - 1] : grad_y(q, lx, ly + 1);
+                    L[q] = sPR[q * A1 + b] - gb * c.dy / 2.0;                    // face[2] of the upper cell
+                    R[q] = sPR[q * A1 + a] + grad_y(q, lx, ly) * c.dy / 2.0;  // face[3] of the lower cell
                 }
                 // momentum roles swapped for y faces (example_Kelvin_Helmholtz/KH_DBA.cu:738-740)
                 kh_rusanov(c, L[0], R[0], L[3], R[3], L[2], R[2], L[1], R[1], &fm, &fe, &fv, &fu);
@@ -341,10 +359,17 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
                     e[3 * TB] = fe;
                 }
             }
-            sGX[0 * A1 + k] = fm;
-            sGX[1 * A1 + k] = fu;
-            sGX[2 * A1 + k] = fv;
-            sGX[3 * A1 + k] = fe;
+            yf[i][0] = fm;
+            yf[i][1] = fu;
+            yf[i][2] = fv;
+            yf[i][3] = fe;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < YS; i++) {
+            const int k = tid + i * NT;
+            if (k >= NF) break;
+            for (int q = 0; q < 4; q++) sP[q * A2 + k] = yf[i][q];
         }
         __syncthreads();
 
@@ -359,8 +384,8 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
             const int fy = lx + (ly + 1) * TB, fym = fy - TB;        // y-face arrays: TB x (TB+1)
             double un[4];
             for (int q = 0; q < 4; q++) {
-                const double ch = dtdx * (-1.0 * sP[q * A2 + fx] + sP[q * A2 + fxm] - 1.0 * sGX[q * A1 + fy] +
-                                          sGX[q * A1 + fym]);
+                const double ch = dtdx * (-1.0 * sXF[q * NF + fx] + sXF[q * NF + fxm] - 1.0 * sP[q * A2 + fy] +
+                                          sP[q * A2 + fym]);
                 const double u0 = I.buf[pself][q][gi] - nb.shift[q][4];
                 un[q] = u0 + ch;
                 part[q] += ch;
