@@ -583,9 +583,13 @@ void launch_dense(const Group* g, hipStream_t s, int first, int step) {
 void launch_finalize(const Group* g, hipStream_t s, int rotate) {
     hipLaunchKernelGGL(finalize_kernel, dim3((unsigned)g->nseg), dim3(kFinNT), 0, s, g->d_insts, g->d_segs,
                        g->nseg, g->d_parts, g->d_counts, g->d_offsets, rotate);
-    if (g->nseg > 1)
-        hipLaunchKernelGGL(finalize_merge_kernel, dim3(1), dim3(kFinNT), 0, s, g->d_insts, g->nseg, g->d_isegs,
-                           (int)g->ids.size(), g->d_parts, g->d_counts, g->d_offsets, rotate);
+    if (g->nseg > 1) {
+        // workgroup 0 scans the segment counts; the others reduce 16 instances each
+        const int n_inst = (int)g->ids.size();
+        const unsigned wgs = 1u + (unsigned)((n_inst + kFinNT / 64 - 1) / (kFinNT / 64));
+        hipLaunchKernelGGL(finalize_merge_kernel, dim3(wgs), dim3(kFinNT), 0, s, g->d_insts, g->nseg, g->d_isegs,
+                           n_inst, g->d_parts, g->d_counts, g->d_offsets, rotate);
+    }
 }
 
 hipEvent_t pooled_event(dba_ctx* ctx) {

@@ -352,11 +352,20 @@ __device__ __forceinline__ void finalize_merge_body(const Inst* __restrict__ ins
                                                     const int32_t* __restrict__ counts,
                                                     int32_t* __restrict__ offsets, int rotate, int* si) {
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    // One workgroup: the instances (a wave each, NT/64 at a time), then the
+    // offsets scan. Several: workgroup 0 scans, workgroup b >= 1 takes
+    // instances [(b-1) NT/64, b NT/64), all at once.
+    const bool one = gridDim.x == 1;
+    int i_lo = 0, i_hi = n_inst;  // this workgroup's instances
+    if (!one) {
+        i_lo = blockIdx.x == 0 ? 0 : ((int)blockIdx.x - 1) * (NT / 64);
+        i_hi = blockIdx.x == 0 ? 0 : min(n_inst, i_lo + NT / 64);
+    }
     // the offsets scan's first counts, loaded with the reductions' inputs
     const int v_first = tid < nseg ? counts[tid] : 0;
-    for (int i0 = 0; i0 < n_inst; i0 += NT / 64) {
+    for (int i0 = i_lo; i0 < i_hi; i0 += NT / 64) {
         const int i = i0 + wid;
-        if (i >= n_inst) break;
+        if (i >= i_hi) break;
         CInst& J = ((CInst*)insts)[i];
         const InstSegs r = n_inst == 1 ? InstSegs{0, nseg} : isegs[i];
         // the instance's scalars and counters, loaded ahead of the reduction
@@ -393,6 +402,7 @@ __device__ __forceinline__ void finalize_merge_body(const Inst* __restrict__ ins
         }
         if (lane == 0) finish_instance(J, ll, q0, q1, qc, cs, vm, step, upd, sc, rotate);
     }
+    if (!one && blockIdx.x != 0) return;
     int running = 0;
     for (int b = 0; b < nseg; b += NT) {
         const int i = b + tid;
