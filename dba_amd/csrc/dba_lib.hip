@@ -90,6 +90,7 @@ struct Instance {
 struct Group {
     std::vector<int> ids;
     int model = 0, tile = 0, tile_z = 1, threads = 0;
+    int dense_threads = 0;        // workgroup size of the single-launch step kernel
     Inst* d_insts = nullptr;
     Seg* d_segs = nullptr;
     InstSegs* d_isegs = nullptr;
@@ -377,6 +378,16 @@ int default_threads(int model, int tile) {
     return model == DBA_MODEL_DENDRITE_K93 ? 512 : 256;
 }
 
+// Single-launch steps of K93 16x16 tiles: every tile takes a workgroup, so a
+// small grid gains from more threads per tile (a shorter FP64 chain per tile)
+// and a large, mostly sleeping one from narrower workgroups (more of them per
+// CU). Measured on MI355X: 512^2 (1024 tiles) 512 threads 8.0 us vs 128 threads
+// 10.0 us; 1024^2 256 threads 9.9 vs 11.8 us; 2048^2 128 threads best.
+int default_dense_threads(int model, int tile, int64_t tiles) {
+    if (model == DBA_MODEL_DENDRITE_K93 && tile == 16) return tiles <= 1024 ? 512 : (tiles <= 4096 ? 256 : 128);
+    return default_threads(model, tile);
+}
+
 // KH: wave speeds of every tile from the current state, and the first CFL dt.
 dba_status kh_prime(dba_ctx* ctx, Instance* in) {
     if (in->p.tile_x == 8)
@@ -428,6 +439,7 @@ dba_status get_group(dba_ctx* ctx, const std::vector<int>& ids, Group** out) {
     g->tile = first->p.tile_x;
     g->tile_z = first->p.tile_z;
     g->threads = first->p.threads ? first->p.threads : default_threads(g->model, g->tile);
+    g->dense_threads = first->p.threads ? first->p.threads : default_dense_threads(g->model, g->tile, g->total_tiles);
     // Model B: 65536 workgroups (4 blocks each on a fully awake 256^3 field)
     // measured faster than 16384 or 3072 with longer pipelines per wave
     int64_t cap = g->model == DBA_MODEL_MODELB ? 65536 : (g->model == DBA_MODEL_DENDRITE_K94 ? 16384 : 4096);
@@ -563,7 +575,7 @@ void launch_dense_tb(const Group* g, hipStream_t s, int first, int step) {
 
 template <int TB>
 void launch_dense_nt(const Group* g, hipStream_t s, int first, int step) {
-    switch (g->threads) {
+    switch (g->dense_threads) {
         case 64: launch_dense_tb<TB, 64>(g, s, first, step); break;
         case 128: launch_dense_tb<TB, 128>(g, s, first, step); break;
         case 512: launch_dense_tb<TB, 512>(g, s, first, step); break;
