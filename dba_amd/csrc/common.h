@@ -153,6 +153,8 @@ struct NbTable {
     int par[2][9];
     int act[2][9];
     double shift[kMaxFields][9];
+    double S;    // single-launch conserved steps: the cumulative shift of this step
+    double tot;  // ... and the tile's change sum
 };
 
 // Owner of the halo cells of tile (ti, tj) (2D) in direction d = (dx+1) + 3 (dy+1):

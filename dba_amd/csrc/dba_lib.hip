@@ -547,18 +547,24 @@ void launch_step(const Group* g, hipStream_t s) {
 // Single-launch steps: one instance in one segment (its whole state fits one
 // canonicalising workgroup), a model with a single-launch kernel, no conserved
 // shift (its per-step global sum needs the finalize).
-bool single_launch_params(const dba_params& p) {
-    return (p.model == DBA_MODEL_CH || p.model == DBA_MODEL_DENDRITE_K93) && p.mode != DBA_MODE_CONSERVED;
+// Conserved DBA needs the previous step's global change sum: single-launch
+// steps recompute it in every workgroup, for one CH instance of <= 1024 tiles.
+bool single_launch_params(const dba_params& p, int64_t nt) {
+    if (p.mode == DBA_MODE_CONSERVED) return p.model == DBA_MODEL_CH && nt <= kSegTiles;
+    return p.model == DBA_MODEL_CH || p.model == DBA_MODEL_DENDRITE_K93;
 }
 
-// Groups whose steps need no global reduction: every instance CH or K93 and not
-// conserved (one instance in one segment, or a tile map for the rest); batches
-// only in mode 2, since every sleeping tile still takes a workgroup.
+// Groups every instance of which is CH or K93 (one instance in one segment, or a
+// tile map for the rest), conserved DBA only as one CH instance in one segment;
+// batches only in mode 2, since every sleeping tile still takes a workgroup.
 bool single_launch_ok(const dba_ctx* ctx, const Group* g) {
     if (ctx->single_launch == 0 || (ctx->single_launch == 1 && g->ids.size() != 1)) return false;
     if (!(g->ids.size() == 1 && g->nseg == 1) && !g->d_dmap) return false;
-    for (int id : g->ids)
-        if (!single_launch_params(ctx->inst.at(id)->p)) return false;
+    for (int id : g->ids) {
+        const Instance* in = ctx->inst.at(id);
+        if (!single_launch_params(in->p, in->h.nt)) return false;
+        if (in->p.mode == DBA_MODE_CONSERVED && !(g->ids.size() == 1 && g->nseg == 1)) return false;
+    }
     return true;
 }
 
@@ -960,7 +966,7 @@ dba_status dba_instance_info(dba_ctx* ctx, int32_t id, dba_info* out) {
     out->last_dt = scal[3];
     out->cons_offset = scal[8];
     out->footprint_bytes = in->footprint;
-    out->single_launch = ctx->single_launch != 0 && single_launch_params(in->p) ? 1 : 0;
+    out->single_launch = ctx->single_launch != 0 && single_launch_params(in->p, in->h.nt) ? 1 : 0;
     out->reserved = 0;
     return DBA_OK;
 }

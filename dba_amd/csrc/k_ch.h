@@ -175,8 +175,12 @@ __device__ __forceinline__ int ch_tile(CInst& I, int t, uint64_t rec, NbTable& n
     if (cons) {
         const double tot = block_sum<NT>(part, sred);
         if (tid == 0) {
-            I.tpart[t] = tot;
-            I.s_tile[(pself ^ 1) * I.nt + t] = I.scal[8];
+            if (WORKLIST) {
+                I.tpart[t] = tot;
+                I.s_tile[(pself ^ 1) * I.nt + t] = I.scal[8];
+            } else {
+                nb.tot = tot;  // the single-launch caller does the bookkeeping
+            }
         }
     }
     __syncthreads();
@@ -227,12 +231,43 @@ __global__ __launch_bounds__(NT) void ch_dense_kernel(const Inst* __restrict__ i
     __shared__ double sred[NT / 64 + 1];
     __shared__ int sflag[NT / 64 + 1];
     __shared__ NbTable nb;
+    __shared__ double sW[kFinNT / 64];
     int t;
     CInst& I = dense_item(insts, dmap, &t);
     const int prev = s & 1, next = prev ^ 1;
     const bool regular = I.mode == 0;
+    const bool cons = I.mode == 2;  // one instance in one segment (the host checks)
     dense_nb_fill(nb, I, t, first, prev, regular, 1);
+    if (cons && !first) {
+        // The previous step's change sums (tpart + (1 + parity) nt, 0 for tiles
+        // that did not run) in the finalize's order: 16 words of 64 tiles, each
+        // by the shuffle tree of block_sum, then the words in order.
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+        const gdouble* tp = I.tpart + (int64_t)(1 + prev) * I.nt;
+        for (int w = wid; w < kFinNT / 64; w += NT / 64) {
+            const int tt = w * 64 + lane;
+            double v = tt < I.nt ? tp[tt] : 0.0;
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) v += __shfl_down(v, off, 64);
+            if (lane == 0) sW[w] = v;
+        }
+    }
     __syncthreads();
+    if (cons) {
+        if (threadIdx.x == 0) {
+            // the cumulative shift of this step, as the finalize advances it
+            double S = I.scal[8];
+            if (!first) {
+                double r = 0.0;
+                for (int w = 0; w < kFinNT / 64; w++) r += sW[w];
+                S = I.scal[12 + prev] + r / I.ntotal;
+            }
+            nb.S = S;
+            for (int d = 0; d < 9; d++) nb.shift[0][d] = S - nb.shift[0][d];  // epoch -> pending shift
+            if (t == 0) I.scal[12 + next] = S;
+        }
+        __syncthreads();
+    }
     const int pself = nb.par[0][4], ran = nb.act[0][4];
     int any = 0;
     if (ran)
@@ -245,6 +280,10 @@ __global__ __launch_bounds__(NT) void ch_dense_kernel(const Inst* __restrict__ i
             I.druns[t] += 1;
             const int64_t step = I.ctr[0] + s;
             if (step < I.hist_cap) atomicAdd((int*)&I.hist[2 * step], 1);
+        }
+        if (cons) {
+            I.tpart[(int64_t)(1 + next) * I.nt + t] = ran ? nb.tot : 0.0;
+            if (ran) I.s_tile[(pself ^ 1) * I.nt + t] = nb.S;
         }
     }
 }

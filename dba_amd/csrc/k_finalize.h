@@ -464,6 +464,9 @@ __device__ __forceinline__ void dense_nb_fill(NbTable& nb, CInst& I, int t, int 
     const int o = nb_owner2d(I, ti, tj, d);
     const int oj = o / ntx, oi = o - oj * ntx;
     nb.tile[d] = o;
+    // conserved: both shift epochs of the owner, loaded beside its parity
+    const bool cons = I.mode == 2;
+    const double e0 = cons ? I.s_tile[o] : 0.0, e1 = cons ? I.s_tile[nt + o] : 0.0;
     for (int s = 0; s < nsets; s++) {
         int par, act;
         if (first) {
@@ -477,7 +480,9 @@ __device__ __forceinline__ void dense_nb_fill(NbTable& nb, CInst& I, int t, int 
         nb.par[s][d] = par;
         nb.act[s][d] = act;
     }
-    for (int f = 0; f < kMaxFields; f++) nb.shift[f][d] = 0.0;
+    // conserved: the epoch of the owner's current buffer (the caller subtracts
+    // it from the step's cumulative shift once that is known)
+    for (int f = 0; f < kMaxFields; f++) nb.shift[f][d] = (cons && f == 0) ? (nb.par[0][d] ? e1 : e0) : 0.0;
 }
 
 // End of a run of n >= 1 single-launch steps: the state the worklist path and
@@ -548,14 +553,23 @@ __device__ __forceinline__ void dense_canon_finish(CInst& I, int n, unsigned lon
     I.scal[3] = dt;
 }
 
-// One instance in one segment: one workgroup does it all.
+// One instance in one segment: one workgroup does it all. Conserved (CH): the
+// cumulative shift after the last step, from the last step's change sums in the
+// finalize's order (block_sum over the tiles, 0 for tiles that did not run).
 __global__ __launch_bounds__(kFinNT) void dense_canon_kernel(const Inst* __restrict__ insts, int n) {
     __shared__ unsigned long long scells[kFinNT / 64];
     __shared__ int scnt[kFinNT / 64];
+    __shared__ double sd[kFinNT / 64 + 1];
     CInst& I = ((CInst*)insts)[0];
     unsigned long long c;
     int cnt;
     dense_canon_seg(I, 0, I.nt, n, scells, scnt, &c, &cnt);
+    if (I.mode == 2) {
+        const int last = n & 1, t = threadIdx.x;
+        const double v = t < I.nt ? I.tpart[(int64_t)(1 + last) * I.nt + t] : 0.0;
+        const double tot = block_sum<kFinNT>(v, sd);
+        if (t == 0) I.scal[8] = I.scal[12 + last] + tot / I.ntotal;
+    }
     if (threadIdx.x == 0) dense_canon_finish(I, n, c, cnt);
 }
 

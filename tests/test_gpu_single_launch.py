@@ -33,7 +33,7 @@ def run(ctx, p, c, single, plan):
         masks.append(inst.active(0)[1])
     info = inst.info()
     out = dict(c=inst.download(0), masks=masks, steps=info.steps_done, updates=info.cell_updates, time=info.sim_time,
-               hist=inst.history(), work=inst.worklist())
+               hist=inst.history(), work=inst.worklist(), shift=info.cons_offset, single=info.single_launch)
     inst.destroy()
     ctx.set_single_launch(True)
     return out
@@ -210,20 +210,24 @@ def test_single_launch_modes(ctx):
     ctx.set_single_launch(True)
 
 
-def test_conserved_uses_the_worklist_path(ctx):
-    """Conserved DBA is not single-launch eligible (its steps need a global sum);
-    it runs (and matches the oracle elsewhere) with the switch on."""
-    n = 64
-    p = configs.ch_params(n=n, tile=16, mode=abi.MODE_CONSERVED, eps=1e-3)
-    c = configs.ch_ic(n, n)
-    a = ctx.create(p)
-    a.upload(0, c)
-    a.step(20, graph_chunk=10)
-    b = ctx.create(p)
-    b.upload(0, c)
-    ctx.set_single_launch(False)
-    b.step(20, graph_chunk=10)
-    ctx.set_single_launch(True)
-    assert np.array_equal(a.download(0), b.download(0))
-    a.destroy()
-    b.destroy()
+@pytest.mark.parametrize("n,tile", [(128, 16), (512, 16), (1024, 16)])
+def test_conserved_single_launch_equals_worklist(ctx, n, tile):
+    """Conserved DBA: one CH instance of <= 1024 tiles takes single-launch steps
+    (each workgroup recomputes the previous step's mean shift in the finalize's
+    order); 1024^2 at 16x16 (4096 tiles) stays on the worklist path. Either way
+    fields, awake sets, counters, history, time and the cumulative shift are
+    bit-identical to the worklist path."""
+    p, c = ch_case(n, tile, abi.MODE_CONSERVED, 9, "periodic")
+    plan = [(1, 0), (4, 0), (30, 10), (3, 0)]
+    a = run(ctx, p, c, False, plan)
+    b = run(ctx, p, c, True, plan)
+    assert np.array_equal(a["c"], b["c"])
+    for x, y in zip(a["masks"], b["masks"]):
+        assert np.array_equal(x, y)
+    assert a["steps"] == b["steps"] and a["updates"] == b["updates"] and a["time"] == b["time"]
+    assert a["shift"] == b["shift"]
+    assert np.array_equal(a["hist"], b["hist"])
+    assert np.array_equal(a["work"], b["work"])
+    assert b["single"] == (1 if (n // tile) ** 2 <= 1024 else 0)
+    # the mean is conserved (to the rounding of the sums)
+    assert abs(a["c"].sum() - c.sum()) < 1e-12 * c.size
