@@ -221,7 +221,7 @@ __global__ __launch_bounds__(NT) void ch_step_kernel(const WorkList wl) {
 // worklist kernel. The canonicalising kernels (dense_canon_*) restore the
 // bitmaps and counters at the end of the run. Results are bit-identical to the
 // worklist path.
-template <int TB, int NT>
+template <int TB, int NT, bool CONS>
 __global__ __launch_bounds__(NT) void ch_dense_kernel(const Inst* __restrict__ insts,
                                                       const int32_t* __restrict__ dmap, int first, int s) {
     constexpr int W2 = TB + 4;
@@ -236,8 +236,8 @@ __global__ __launch_bounds__(NT) void ch_dense_kernel(const Inst* __restrict__ i
     CInst& I = dense_item(insts, dmap, &t);
     const int prev = s & 1, next = prev ^ 1;
     const bool regular = I.mode == 0;
-    const bool cons = I.mode == 2;  // one instance in one segment (the host checks)
-    dense_nb_fill(nb, I, t, first, prev, regular, 1);
+    constexpr bool cons = CONS;  // conserved DBA: one instance in one segment (the host checks)
+    dense_nb_fill(nb, I, t, first, prev, regular, 1, cons);
     if (cons && !first) {
         // The previous step's change sums (tpart + (1 + parity) nt, 0 for tiles
         // that did not run) in the finalize's order: 16 words of 64 tiles, each

@@ -457,7 +457,7 @@ __device__ __forceinline__ CInst& dense_item(const Inst* insts, const int32_t* d
 // (awake = the 5-point dilation of its detector bits, the finalize's rule).
 // dF / dP hold, per step parity, nsets x nt bytes (set-major).
 __device__ __forceinline__ void dense_nb_fill(NbTable& nb, CInst& I, int t, int first, int prev, bool regular,
-                                              int nsets) {
+                                              int nsets, bool cons) {
     const int d = threadIdx.x;
     if (d >= 9) return;
     const int nt = I.nt, ntx = I.ntx, tj = t / ntx, ti = t - tj * ntx;
@@ -465,7 +465,6 @@ __device__ __forceinline__ void dense_nb_fill(NbTable& nb, CInst& I, int t, int 
     const int oj = o / ntx, oi = o - oj * ntx;
     nb.tile[d] = o;
     // conserved: both shift epochs of the owner, loaded beside its parity
-    const bool cons = I.mode == 2;
     const double e0 = cons ? I.s_tile[o] : 0.0, e1 = cons ? I.s_tile[nt + o] : 0.0;
     for (int s = 0; s < nsets; s++) {
         int par, act;

@@ -195,7 +195,7 @@ __global__ __launch_bounds__(NT) void k93_dense_kernel(const Inst* __restrict__ 
     const int nt = I.nt;
     const int prev = s & 1, next = prev ^ 1;
     const bool regular = I.mode == 0;
-    dense_nb_fill(nb, I, t, first, prev, regular, 2);
+    dense_nb_fill(nb, I, t, first, prev, regular, 2, false);
     __syncthreads();
     const int ap = nb.act[0][4], at = nb.act[1][4];
     int bits = 0;
