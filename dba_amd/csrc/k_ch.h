@@ -242,14 +242,21 @@ __global__ __launch_bounds__(NT) void ch_dense_kernel(const Inst* __restrict__ i
         // The previous step's change sums (tpart + (1 + parity) nt, 0 for tiles
         // that did not run) in the finalize's order: 16 words of 64 tiles, each
         // by the shuffle tree of block_sum, then the words in order.
+        constexpr int NWV = NT / 64, PER = (kFinNT / 64 + NWV - 1) / NWV;
         const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
         const gdouble* tp = I.tpart + (int64_t)(1 + prev) * I.nt;
-        for (int w = wid; w < kFinNT / 64; w += NT / 64) {
-            const int tt = w * 64 + lane;
-            double v = tt < I.nt ? tp[tt] : 0.0;
+        double v[PER];
 #pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) v += __shfl_down(v, off, 64);
-            if (lane == 0) sW[w] = v;
+        for (int k = 0; k < PER; k++) {  // all of this wave's words in flight at once
+            const int w = wid + k * NWV, tt = w * 64 + lane;
+            v[k] = (w < kFinNT / 64 && tt < I.nt) ? tp[tt] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+            const int w = wid + k * NWV;
+            if (lane == 0 && w < kFinNT / 64) sW[w] = v[k];
         }
     }
     __syncthreads();
