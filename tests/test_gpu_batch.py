@@ -92,18 +92,19 @@ def run_alone(ctx, r, tiles):
     return out, upd
 
 
-@pytest.mark.parametrize("chunk", [100, 7])
-def test_serving_result_same_alone_and_in_mix(ctx, chunk):
+@pytest.mark.parametrize("chunk,modes", [(100, None), (7, None), (13, {"ch": 2, "dendrite": 1})])
+def test_serving_result_same_alone_and_in_mix(ctx, chunk, modes):
     """Staggered arrivals under a tight budget: requests join and leave the
     resident set at different ticks (chunk 7 also drives the graph and group
-    caches through eviction); each result equals its solo run bit for bit."""
+    caches through eviction; the per-kind modes step the CH batch with one
+    launch per step); each result equals its solo run bit for bit."""
     rs = serve.make_requests(10, master_seed=21, side_lo=64, side_hi=256, steps_lo=30, steps_hi=400)
     for k, r in enumerate(rs):
         r.arrival = 0.002 * k
     tiles = {"dendrite": 16, "ch": 32}
     be = serve.LibraryBackend(ctx)
     cap = 3 * max(ctx.footprint(r.params(tiles, {})) for r in rs)
-    srv = serve.Server(be, cap, chunk=chunk, tiles=tiles, keep_fields=True)
+    srv = serve.Server(be, cap, chunk=chunk, tiles=tiles, keep_fields=True, modes=modes)
     done = srv.run(rs)
     assert sorted(c.rid for c in done) == list(range(10))
     assert srv.max_used <= cap
