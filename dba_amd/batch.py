@@ -112,12 +112,14 @@ def run_batched(ctx, items, steps, tile=16, threads=0, chunk=100, fields_cache=N
     ctx.sync()
     u0 = sum(i.info().cell_updates for i in insts)
     ctx.set_single_launch(single_launch)
-    ctx.marker(0)
-    t0 = time.perf_counter()
-    ctx.step(insts, steps, graph_chunk=min(chunk, steps) if steps else 0)
-    ctx.marker(1)
-    ctx.sync()
-    ctx.set_single_launch(1)
+    try:
+        ctx.marker(0)
+        t0 = time.perf_counter()
+        ctx.step(insts, steps, graph_chunk=min(chunk, steps) if steps else 0)
+        ctx.marker(1)
+        ctx.sync()
+    finally:
+        ctx.set_single_launch(1)
     wall = time.perf_counter() - t0
     gpu_ms = ctx.elapsed_ms(0, 1)
     u1 = sum(i.info().cell_updates for i in insts)
@@ -185,27 +187,29 @@ def autotune(ctx, items, trial_steps=200, warm_steps=20, tiles=TILES, threads=TH
     if fields_cache is None:
         fields_cache = {it.index: it.fields() for it in items}
     table = []
-    for tile in tiles:
-        if any(it.side % tile for it in items):
-            continue
-        for nt in threads:
-            for mode in modes:
-                ctx.set_single_launch(mode)
-                insts = _create(ctx, items, tile, nt, fields_cache)
-                ctx.step(insts, warm_steps, graph_chunk=min(chunk, warm_steps))
-                ctx.sync()
-                u0 = sum(i.info().cell_updates for i in insts)
-                t0 = time.perf_counter()
-                ctx.step(insts, trial_steps, graph_chunk=min(chunk, trial_steps))
-                ctx.sync()
-                dt = time.perf_counter() - t0
-                u1 = sum(i.info().cell_updates for i in insts)
-                table.append((tile, nt, mode, dt * 1e3 / trial_steps, (u1 - u0) / dt))
-                if log:
-                    log(f"autotune tile {tile} threads {nt} mode {mode}: {dt * 1e3 / trial_steps:.4f} ms/step")
-                for i in insts:
-                    i.destroy()
-    ctx.set_single_launch(1)
+    try:
+        for tile in tiles:
+            if any(it.side % tile for it in items):
+                continue
+            for nt in threads:
+                for mode in modes:
+                    ctx.set_single_launch(mode)
+                    insts = _create(ctx, items, tile, nt, fields_cache)
+                    ctx.step(insts, warm_steps, graph_chunk=min(chunk, warm_steps))
+                    ctx.sync()
+                    u0 = sum(i.info().cell_updates for i in insts)
+                    t0 = time.perf_counter()
+                    ctx.step(insts, trial_steps, graph_chunk=min(chunk, trial_steps))
+                    ctx.sync()
+                    dt = time.perf_counter() - t0
+                    u1 = sum(i.info().cell_updates for i in insts)
+                    table.append((tile, nt, mode, dt * 1e3 / trial_steps, (u1 - u0) / dt))
+                    if log:
+                        log(f"autotune tile {tile} threads {nt} mode {mode}: {dt * 1e3 / trial_steps:.4f} ms/step")
+                    for i in insts:
+                        i.destroy()
+    finally:
+        ctx.set_single_launch(1)
     best = min(table, key=lambda r: r[3])
     return (best[0], best[1], best[2]), table
 

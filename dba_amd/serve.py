@@ -136,10 +136,13 @@ class LibraryBackend:
         return inst
 
     def step(self, handles, n, chunk, mode=None):
-        if mode is not None:
-            self.ctx.set_single_launch(mode)
-        self.ctx.step(handles, n, graph_chunk=chunk)
-        if mode is not None:
+        if mode is None:
+            self.ctx.step(handles, n, graph_chunk=chunk)
+            return
+        self.ctx.set_single_launch(mode)
+        try:
+            self.ctx.step(handles, n, graph_chunk=chunk)
+        finally:
             self.ctx.set_single_launch(1)
 
     def sync(self):
