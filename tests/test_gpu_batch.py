@@ -22,10 +22,15 @@ def fields_of(inst):
     return [inst.download(0), inst.download(1)]
 
 
-def test_batched_equals_alone(ctx):
-    items = small_sweep()
+@pytest.mark.parametrize("n", [6, 36])
+def test_batched_equals_alone(ctx, n):
+    """36 instances: the finalize merge takes three workgroups of instance
+    reductions (16 instances each) beside the offsets scan."""
+    items = small_sweep(n)
     st, insts = batch.run_batched(ctx, items, 300, tile=16, chunk=100, keep=True)
     assert st.active_updates > 0
+    infos = [inst.info() for inst in insts]
+    assert sum(i.steps_done for i in infos) == 300 * n
     for it, inst in zip(items, insts):
         (alone,) = batch._create(ctx, [it], 16, 0)
         alone.step(300, graph_chunk=100)
@@ -33,6 +38,10 @@ def test_batched_equals_alone(ctx):
             assert np.array_equal(a, b), it.index
         assert np.array_equal(inst.active(0)[1], alone.active(0)[1])
         assert np.array_equal(inst.active(1)[1], alone.active(1)[1])
+        ai = alone.info()
+        bi = inst.info()
+        assert (bi.steps_done, bi.cell_updates, bi.sim_time) == (ai.steps_done, ai.cell_updates, ai.sim_time)
+        assert np.array_equal(inst.history(), alone.history())
         alone.destroy()
         inst.destroy()
 
