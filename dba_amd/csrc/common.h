@@ -109,13 +109,36 @@ __device__ __forceinline__ int block_any(int pred, int* lds_scratch) {
     return r;
 }
 
-// Deterministic workgroup sum: per-wave butterfly-free shift reduction in a
-// fixed order, then wave totals added in wave order by thread 0.
+// Lane i gets v of lane i + N in its row of 16 lanes (a DPP row shift, no LDS
+// traffic); lanes whose source leaves the row get 0.
+template <int N>
+__device__ __forceinline__ double dpp_row_down(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x100 + N, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x100 + N, 0xF, 0xF, true);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Lane 0 gets v summed over the wave by the shift tree v[i] += v[i + off],
+// off = 32, 16, 8, 4, 2, 1: the two levels that cross 16-lane rows by lane
+// permutes, the four inside a row by DPP row shifts. Lane 0 sees the operands
+// of a plain shuffle tree in the same order, so the sum is the same double.
+__device__ __forceinline__ double wave_sum_down(double v) {
+    v += __shfl_down(v, 32, 64);
+    v += __shfl_down(v, 16, 64);
+    v += dpp_row_down<8>(v);
+    v += dpp_row_down<4>(v);
+    v += dpp_row_down<2>(v);
+    v += dpp_row_down<1>(v);
+    return v;
+}
+
+// Deterministic workgroup sum: per-wave shift reduction in a fixed order, then
+// wave totals added in wave order by thread 0.
 template <int NT>
 __device__ __forceinline__ double block_sum(double v, double* lds_scratch) {
     constexpr int NW = NT / 64;
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_down(v, off, 64);
+    v = wave_sum_down(v);
     int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) lds_scratch[wid] = v;
     __syncthreads();

@@ -253,8 +253,7 @@ __global__ __launch_bounds__(NT) void ch_dense_kernel(const Inst* __restrict__ i
         }
 #pragma unroll
         for (int k = 0; k < PER; k++) {
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+            v[k] = wave_sum_down(v[k]);  // block_sum's tree
             const int w = wid + k * NWV;
             if (lane == 0 && w < kFinNT / 64) sW[w] = v[k];
         }
