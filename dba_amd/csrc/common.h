@@ -236,8 +236,13 @@ __device__ __forceinline__ int nb_dir1(int l) {
 template <int NT>
 __device__ __forceinline__ double block_max(double v, double* lds_scratch) {
     constexpr int NW = NT / 64;
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+    // the shift tree of wave_sum_down with fmax (lane 0 sees the same operands)
+    v = fmax(v, __shfl_down(v, 32, 64));
+    v = fmax(v, __shfl_down(v, 16, 64));
+    v = fmax(v, dpp_row_down<8>(v));
+    v = fmax(v, dpp_row_down<4>(v));
+    v = fmax(v, dpp_row_down<2>(v));
+    v = fmax(v, dpp_row_down<1>(v));
     int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) lds_scratch[wid] = v;
     __syncthreads();

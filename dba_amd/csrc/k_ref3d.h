@@ -499,9 +499,7 @@ __global__ __launch_bounds__(64) void modelb_update_kernel(const WorkList wl) {
                 part[2] = chz;
             }
             for (int f = 0; f < ncons; f++) {
-                double tot = part[f];
-#pragma unroll
-                for (int off = 32; off >= 1; off >>= 1) tot += __shfl_down(tot, off, 64);
+                const double tot = wave_sum_down(part[f]);
                 if (lane == 0) {
                     I.tpart[(int64_t)f * I.nt + b.t] = tot;
                     I.s_tile[((int64_t)f * 2 + (pself ^ 1)) * I.nt + b.t] = I.scal[8 + f];
