@@ -20,9 +20,15 @@
  *   k94_*     pinned to float32 resolution against the compiled (float32)
  *             reference CPU program on the central block.
  *   DBA modes (all models): always-awake equals regular bitwise, a tile that
- *             never fires stays frozen bitwise, conserved totals invariant; the
- *             stale-neighbour reads follow the reference's CUDA kernels, which
- *             cannot run here, so they are only partially pinned.
+ *             never fires stays frozen bitwise, conserved totals invariant, and
+ *             the stale-neighbour data flow pinned on tiny grids with prescribed
+ *             awake sets against numpy restatements of the reference's kernel
+ *             sequences (tests/dba_sequence.py, tests/test_oracle_stale.py):
+ *             stale mu (CH, Model B), dropped latent heat in p-awake/T-asleep
+ *             tiles (K93, K94), stale primitives/gradients/primes and face
+ *             fluxes (KH), the mean shift on every cell (conserved CH, Model B,
+ *             KH). Each of those tests also builds this file with one planted
+ *             mistake (ORA_MUTANT below) and checks that the mistake is caught.
  *   dilate / compact  pinned: brute force over all small flag patterns.
  */
 #define _GNU_SOURCE
@@ -35,6 +41,19 @@
 
 #define MAXF 4
 #define MAXW 32
+
+/* Planted mistakes for the tests that pin the DBA data flow
+   (tests/test_oracle_stale.py compiles copies with -DORA_MUTANT=k and checks that
+   each is caught). The default build, the only one anything else loads, has none.
+     1  chemical potential recomputed on every cell every step (fresh, not the
+        value stored when a sleeping tile last ran) - CH, Model B
+     2  latent heat applied and kept in p-awake / T-asleep tiles - K93, K94
+     3  KH primitives, gradients and primes recomputed on every cell every step
+     4  KH face fluxes recomputed on every cell every step
+     5  conserved mean shift spread over the awake cells only - CH, Model B, KH */
+#ifndef ORA_MUTANT
+#define ORA_MUTANT 0
+#endif
 
 struct ora_sim {
     dba_params p;
@@ -164,7 +183,7 @@ static void ch_calc_mu(ora_sim* s, int y0, int y1) {
     double* metric = s->w[1];
     for (int y = y0; y < y1; y++)
         for (int x = 0; x < s->nx; x++) {
-            if (!active(s, 0, x, y, 0)) continue;
+            if (!active(s, 0, x, y, 0) && ORA_MUTANT != 1) continue;
             long i = (long)x + (long)y * s->nx;
             double lc = ch_lap(s, c, x, y);
             double ci = c[i];
@@ -206,9 +225,20 @@ static void ch_step(ora_sim* s) {
            full swap (example_Reactive_ModelB/Conserved/kernel.cu:1152-1173). */
         double sum = 0.0;
         for (long i = 0; i < n; i++) sum += s->w[2][i];
+#if ORA_MUTANT == 5
+        long na = 0;
+        for (int y = 0; y < s->ny; y++)
+            for (int x = 0; x < s->nx; x++) na += active(s, 0, x, y, 0);
+        for (int y = 0; y < s->ny; y++)
+            for (int x = 0; x < s->nx; x++) {
+                long i = (long)x + (long)y * s->nx;
+                if (active(s, 0, x, y, 0)) s->f[0][i] = s->fn[0][i] - sum / (double)na;
+            }
+#else
         double avg = sum / (double)n;
         for (long i = 0; i < n; i++) s->fn[0][i] -= avg;
         memcpy(s->f[0], s->fn[0], sizeof(double) * n);
+#endif
     } else {
         for (int y = 0; y < s->ny; y++)
             for (int x = 0; x < s->nx; x++)
@@ -314,6 +344,11 @@ static void k93_evolve(ora_sim* s, int y0, int y1) {
                 Tn[i] = ap ? t1 + s->p.latent_k * (pn[i] - pi_) : t1;
                 s->w[5][i] = fabs(Tn[i] - T[i]);
             }
+#if ORA_MUTANT == 2
+            else if (ap) {
+                Tn[i] = T[i] + s->p.latent_k * (pn[i] - pi_);
+            }
+#endif
         }
 }
 
@@ -342,6 +377,11 @@ static void k93_step(ora_sim* s) {
                 s->f[1][i] = s->fn[1][i];
                 if (s->w[5][i] >= mT) s->next[1][t] = 1;
             }
+#if ORA_MUTANT == 2
+            else if (s->awake[0][t]) {
+                s->f[1][i] = s->fn[1][i];
+            }
+#endif
         }
     dilate_set(s, 0);
     dilate_set(s, 1);
@@ -505,7 +545,7 @@ static void kh_step(ora_sim* s) {
     /* 1. primitives on awake blocks (getPrimitive_wakeup) */
     for (int y = 0; y < s->ny; y++)
         for (int x = 0; x < s->nx; x++) {
-            if (!active(s, 0, x, y, 0)) continue;
+            if (!active(s, 0, x, y, 0) && ORA_MUTANT != 3) continue;
             long i = (long)x + (long)y * s->nx, c = pidx(s, x, y);
             kh_prims_of(s, i, &s->w[0][c], &s->w[1][c], &s->w[2][c], &s->w[3][c]);
         }
@@ -526,13 +566,13 @@ static void kh_step(ora_sim* s) {
     /* 2. gradients, primes, metric on awake blocks (extrap_in_time_wakeup) */
     for (int y = 0; y < s->ny; y++)
         for (int x = 0; x < s->nx; x++)
-            if (active(s, 0, x, y, 0)) kh_grads_at(s, x, y);
+            if (active(s, 0, x, y, 0) || ORA_MUTANT == 3) kh_grads_at(s, x, y);
     kh_fill_grad_ghosts(s);
     /* 3. fluxes on awake blocks (calc_flux_wakeup); walls: the bottom face of row 0 */
     int wall = s->p.bc_y == DBA_BC_WALL;
     for (int y = 0; y < s->ny; y++)
         for (int x = 0; x < s->nx; x++) {
-            if (!active(s, 0, x, y, 0)) continue;
+            if (!active(s, 0, x, y, 0) && ORA_MUTANT != 4) continue;
             kh_face_fluxes(s, x, y, 1, 1);
             if (wall && y == 0) kh_face_fluxes(s, x, -1, 0, 1);
         }
@@ -562,8 +602,17 @@ static void kh_step(ora_sim* s) {
         for (int q = 0; q < 4; q++) {
             double sum = 0.0;
             for (long i = 0; i < s->n; i++) sum += s->w[24 + q][i];
+#if ORA_MUTANT == 5
+            long na = 0;
+            for (int y = 0; y < s->ny; y++)
+                for (int x = 0; x < s->nx; x++) na += active(s, 0, x, y, 0);
+            for (int y = 0; y < s->ny; y++)
+                for (int x = 0; x < s->nx; x++)
+                    if (active(s, 0, x, y, 0)) s->f[q][(long)x + (long)y * s->nx] -= sum / (double)na;
+#else
             double avg = sum / (double)s->n;
             for (long i = 0; i < s->n; i++) s->f[q][i] -= avg;
+#endif
         }
     }
     /* 5. Wakeup_Next: any(metric > Metric_eps), then 3x3 dilation */
@@ -633,6 +682,11 @@ static void k94_step(ora_sim* s) {
                         s->w[0][i] = fabs(tn - T[i]);
                     }
                 }
+#if ORA_MUTANT == 2
+                else if (dba && ap) {
+                    Tn[i] = T[i] + (pn[i] - p[i]);
+                }
+#endif
             }
     if (!dba) {
         memcpy(s->f[0], s->fn[0], sizeof(double) * s->n);
@@ -655,6 +709,11 @@ static void k94_step(ora_sim* s) {
                     s->f[1][i] = Tn[i];
                     if (s->w[0][i] >= mT) s->next[1][t] = 1;
                 }
+#if ORA_MUTANT == 2
+                else if (s->awake[0][t]) {
+                    s->f[1][i] = Tn[i];
+                }
+#endif
             }
     dilate_set(s, 0);
     dilate_set(s, 1);
@@ -685,7 +744,7 @@ static void mb_step(ora_sim* s) {
     for (int z = 0; z < s->nz; z++)
         for (int y = 0; y < s->ny; y++)
             for (int x = 0; x < s->nx; x++) {
-                if (!active(s, 0, x, y, z)) continue;
+                if (!active(s, 0, x, y, z) && ORA_MUTANT != 1) continue;
                 long i = (long)x + (long)y * s->nx + (long)z * s->nx * s->ny;
                 B[i] = 1.0 - X[i] - Y[i] - Z[i];
                 dfx[i] = -1.0 + 1.0 / P->r_x + P->chi_xy * Y[i] + P->chi_xz * Z[i] + P->chi_xb * B[i] -
@@ -728,6 +787,23 @@ static void mb_step(ora_sim* s) {
         return;
     }
     if (cons) {
+#if ORA_MUTANT == 5
+        long na = 0;
+        for (int z = 0; z < s->nz; z++)
+            for (int y = 0; y < s->ny; y++)
+                for (int x = 0; x < s->nx; x++) na += active(s, 0, x, y, z);
+        for (int k = 0; k < 3; k++) {
+            double sum = 0.0;
+            for (long i = 0; i < s->n; i++) sum += s->w[4 + k][i];
+            for (int z = 0; z < s->nz; z++)
+                for (int y = 0; y < s->ny; y++)
+                    for (int x = 0; x < s->nx; x++)
+                        if (active(s, 0, x, y, z)) {
+                            long i = (long)x + (long)y * s->nx + (long)z * s->nx * s->ny;
+                            s->f[k][i] = s->fn[k][i] - sum / (double)na;
+                        }
+        }
+#else
         for (int k = 0; k < 3; k++) {
             double sum = 0.0;
             for (long i = 0; i < s->n; i++) sum += s->w[4 + k][i];
@@ -735,6 +811,7 @@ static void mb_step(ora_sim* s) {
             for (long i = 0; i < s->n; i++) s->fn[k][i] -= avg;
         }
         for (int k = 0; k < 3; k++) memcpy(s->f[k], s->fn[k], sizeof(double) * s->n);
+#endif
     } else {
         for (int z = 0; z < s->nz; z++)
             for (int y = 0; y < s->ny; y++)

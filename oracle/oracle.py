@@ -22,11 +22,16 @@ def _params_type():
 
 def load():
     global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
-        raise OSError(f"{LIB_PATH} missing: build it with __graft_entry__.build()")
-    lib = C.CDLL(LIB_PATH)
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} missing: build it with __graft_entry__.build()")
+        _lib = load_path(LIB_PATH)
+    return _lib
+
+
+def load_path(path):
+    """The oracle built at `path` (tests load copies built with a planted mistake)."""
+    lib = C.CDLL(path)
     P = C.POINTER(_params_type())
     lib.ora_create.restype = C.c_void_p
     lib.ora_create.argtypes = [P]
@@ -49,15 +54,14 @@ def load():
     lib.ora_compact.restype = C.c_int
     lib.ora_bench.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.ora_bench.restype = C.c_double
-    _lib = lib
     return lib
 
 
 class OracleSim:
     """One oracle simulation with the dba_params of the GPU instance it checks."""
 
-    def __init__(self, params):
-        self.lib = load()
+    def __init__(self, params, lib=None):
+        self.lib = lib if lib is not None else load()
         self.params = params.copy()
         self.h = self.lib.ora_create(C.byref(self.params))
         if not self.h:

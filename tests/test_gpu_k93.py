@@ -44,6 +44,42 @@ def test_k93_matches_oracle(ctx, mode, tile, threads):
     inst.destroy()
 
 
+@pytest.mark.parametrize("single_launch", [1, 0])
+@pytest.mark.parametrize("tile,threads", [(16, 0), (32, 256)])
+def test_k93_latent_heat_dropped_matches_oracle(ctx, tile, threads, single_launch):
+    """Metric_T = 0.03 puts the T set to sleep while the interface keeps the p
+    set awake: about 1700 tile-steps are p-awake and T-asleep, where the
+    reference drops the latent heat (example_Dendritic_growth/kernel.cu:411-574).
+    An oracle that kept it ends 1.06 apart (tests/test_oracle_stale.py)."""
+    n = 128
+    p = configs.dendrite_k93_params(n=n, tile=tile, threads=threads)
+    p.thresh_b = 0.03
+    phi, T = configs.dendrite_k93_ic(n, n)
+    ctx.set_single_launch(single_launch)
+    try:
+        inst = ctx.create(p)
+        inst.upload(0, phi)
+        inst.upload(1, T)
+        o = OracleSim(p)
+        o.set_field(0, phi)
+        o.set_field(1, T)
+        dropped = 0
+        for chunk in (1, 9, 40, 100):
+            ap, at = o.awake(0), o.awake(1)
+            dropped += int((ap & (1 - at)).sum())
+            inst.step(chunk)
+            o.step(chunk)
+            for f in (0, 1):
+                d = np.max(np.abs(inst.download(f) - o.get_field(f)))
+                assert d <= TOL, (f, d)
+            for s in (0, 1):
+                assert np.array_equal(inst.active(s)[1], o.awake(s))
+        assert dropped > 0
+        inst.destroy()
+    finally:
+        ctx.set_single_launch(1)
+
+
 @pytest.mark.parametrize("j,theta0,bc", [(4, 0.0, abi.BC_NOFLUX), (6, 0.2, abi.BC_PERIODIC)])
 def test_k93_active_sets_bit_exact(ctx, j, theta0, bc):
     n, tile = 128, 16

@@ -101,6 +101,25 @@ def test_k94_matches_oracle(ctx, mode, nz):
     inst.destroy()
 
 
+def test_k94_latent_heat_dropped_matches_oracle(ctx):
+    """Metric_T = 0.01: p-awake / T-asleep blocks occur (about 3500 block-steps
+    over 36 steps), where the reference drops the latent heat
+    (example_Dendritic_growth/kernel.cu:411-574)."""
+    n = 32
+    p0, T0 = configs.dendrite_k94_ic(n, n, n)
+    p = configs.dendrite_k94_params(n=n, mode=abi.MODE_DBA, thresh_b=0.01)
+    inst, o = pair(ctx, p, [p0, T0])
+    for chunk in (1, 5, 30):
+        inst.step(chunk)
+        o.step(chunk)
+        for f in (0, 1):
+            d = np.max(np.abs(inst.download(f) - o.get_field(f)))
+            assert d <= TOL, (f, d)
+        for s in (0, 1):
+            assert np.array_equal(inst.active(s)[1], o.awake(s))
+    inst.destroy()
+
+
 def test_modelb_reference_size_runs(ctx):
     """64^3 blocks of the reference Input.txt layout, 50 steps through a graph."""
     n = 64
