@@ -25,12 +25,20 @@ kernel), replayed from a captured hipGraph. Inputs are resident in HBM before
 the timed region. value = active cell-updates/s over the timed steps (device
 counter of cells in awake tiles).
 
+Before the timed region the graph it replays is captured, instantiated and
+replayed once, whatever --warmup is, so the timed region only replays it.
+
 roofline: the step kernel's algorithmic bytes per launch (B_alg x active cells
-of that launch: CH fused pass 16 B = read c + write c'; dendrite 32 B = read and
-write phi and T; KH 64 B = read and write the 4 conserved fields) over its average duration, timed with hipEvents on the
-library stream in a second, eager pass of the same workload.
-cpu_baseline: the oracle's regular-mode loop (serial and y-slab threads on the
-host cores) on a bounded sample of the same grid.
+of that launch: CH fused pass 16 B = read c + write c'; dendrite 16 B per (cell,
+field) update; KH 64 B = read and write the 4 conserved fields) over its average
+duration. One launch per step (single-launch steps): the GPU time per step of a
+separate graph-replayed run of at least --timing-steps steps, between hipEvents
+on the library stream. Two launches per step: hipEvents around every step
+kernel in an eager pass.
+cpu_baseline: the oracle's regular-mode loop on a bounded sample of the same
+grid: serial, y-slab threads on the job's CPU share, and for CH the MPI slab
+restatement (oracle/mpi_slab.c) on as many ranks; value is the faster all-core
+leg, with the core counts and the CPU model reported beside it.
 """
 import argparse
 import json
@@ -83,6 +91,16 @@ def make(workload, n=None, tile=None, threads=0):
         p = configs.dendrite_k93_params(n=w["n"], tile=w["tile"], mode=mode, threads=threads)
         fields = list(configs.dendrite_k93_ic(w["n"], w["n"]))
     return p, fields
+
+
+def graph_chunk_for(K, cap):
+    """Steps per captured graph for a K-step timed region: the largest divisor of K
+    not above cap, so the timed region is whole replays of one cached graph."""
+    cap = max(1, min(cap, K)) if K > 0 else 1
+    for c in range(cap, 0, -1):
+        if K % c == 0:
+            return c
+    return 1
 
 
 def reference_cpu_baseline(workload, budget_s=15.0):
@@ -144,9 +162,58 @@ def reference_cpu_baseline(workload, budget_s=15.0):
                       f"({label}, g++ -O2), 1 thread ({cpu_model()})"}
 
 
-def cpu_baseline(workload, budget_s=12.0, n=None):
-    """Oracle regular-mode loop on the host cores, bounded sample (the reference's
-    own CPU program for the 3D reference workloads when it is built)."""
+def host_cores():
+    """(cores used, facts): the CPU share this process may use. The affinity mask
+    can list more CPUs than the job's share; OMP_NUM_THREADS, when set, states the
+    share (the GPU box sets it), and the smaller of the two is used."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    cores = min(aff, int(share)) if share.isdigit() and int(share) > 0 else aff
+    facts = {"affinity_cpus": aff, "nproc": os.cpu_count(), "omp_num_threads": share or None,
+             "cpu_model": cpu_model(), "cpu_quota": cgroup_cpu_quota()}
+    return max(1, cores), facts
+
+
+def cgroup_cpu_quota():
+    """CPUs' worth of the cgroup v2 quota (cpu.max), or None without one."""
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        return None if q == "max" else int(q) / int(per)
+    except (OSError, ValueError):
+        return None
+
+
+def mpirun_path():
+    import shutil
+    return shutil.which("mpirun") or ("/opt/conda/bin/mpirun" if os.path.exists("/opt/conda/bin/mpirun") else None)
+
+
+def mpi_slab_baseline(p, field, ranks, steps):
+    """The MPI slab restatement (oracle/mpi_slab.c: y-slabs, periodic MPI_Sendrecv
+    ring of halo rows, the reference MPI_version's decomposition) on `ranks`
+    processes; None when MPICH or the program is absent."""
+    import subprocess
+    import tempfile
+    import numpy as np
+    exe = os.path.join(ROOT, "oracle", "_build", "mpi_slab")
+    mr = mpirun_path()
+    if mr is None or not os.path.exists(exe):
+        return None
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "p.bin"), "wb") as f:
+            f.write(bytes(p))
+        np.ascontiguousarray(field, dtype=np.float64).tofile(os.path.join(d, "ic.bin"))
+        r = subprocess.run([mr, "-np", str(ranks), exe, os.path.join(d, "p.bin"), os.path.join(d, "ic.bin"),
+                            str(steps)], capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        return None
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def cpu_baseline(workload, budget_s=18.0, n=None):
+    """Oracle regular-mode loop on the host cores, bounded sample: serial, y-slab
+    threads and (CH) the MPI slab restatement on every core of the CPU share; the
+    reference's own CPU program for the 3D reference workloads when it is built."""
     from dba_amd import abi
     from oracle.oracle import OracleSim
     if n is None:
@@ -156,31 +223,48 @@ def cpu_baseline(workload, budget_s=12.0, n=None):
     p, fields = make(workload, n=n)
     p.mode = abi.MODE_REGULAR
     ncell = p.nx * p.ny * p.nz
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    cores = max(1, min(cores, 16))
-    if p.nz > 1:
-        cores = 1  # the oracle steps the 3D models serially
+    cores, facts = host_cores()
+    nth_all = cores if p.nz == 1 else 1  # the oracle steps the 3D models serially
+    per = budget_s / 3
     res = {}
-    for label, nth in (("serial", 1), ("threads", cores)):
+    for label, nth in (("serial", 1), ("threads", nth_all)):
         if label == "threads" and nth == 1:
             res[label] = res["serial"]
             continue
         o = OracleSim(p)
         for f, a in enumerate(fields):
             o.set_field(f, a)
-        t = o.bench(1 if p.nz > 1 else 2, nth)  # calibrate
-        steps = max(1, int(budget_s / 2 / max(t / (1 if p.nz > 1 else 2), 1e-6)))
-        steps = min(steps, 2000)
+        k0 = 1 if p.nz > 1 else 2
+        t = o.bench(k0, nth)  # calibrate
+        steps = max(1, min(2000, int(per / max(t / k0, 1e-6))))
         t = o.bench(steps, nth)
         res[label] = (ncell * steps / t, steps, nth)
         o.close()
-    v, steps, nth = res["threads"]
-    return {
-        "value": v, "unit": "cell-updates/s", "cores": nth, "kind": "port",
-        "sample": f"{steps} regular-mode steps of the {p.nx}x{p.ny}x{p.nz} grid, {nth} y-slab threads ({cpu_model()})",
+    out = {
+        "value": res["threads"][0], "unit": "cell-updates/s", "cores": res["threads"][2], "kind": "port",
+        "sample": f"{res['threads'][1]} regular-mode steps of the {p.nx}x{p.ny}x{p.nz} grid, "
+                  f"{res['threads'][2]} y-slab threads ({facts['cpu_model']})",
         "serial_value": res["serial"][0],
         "serial_sample": f"{res['serial'][1]} regular-mode steps, 1 thread",
+        "threads_value": res["threads"][0],
+        "host": facts,
     }
+    if p.model == abi.MODEL_CH and p.mode == abi.MODE_REGULAR and nth_all > 1:
+        steps = max(2, min(5000, int(per * res["threads"][0] / ncell)))
+        m = mpi_slab_baseline(p, fields[0], nth_all, steps)
+        if m is not None:
+            out["mpi_value"] = m["cell_updates_per_s"]
+            out["mpi_sample"] = (f"{m['steps']} regular-mode steps, {m['ranks']} MPI ranks in y-slabs "
+                                 "(oracle/mpi_slab.c, MPICH)")
+            if m["cell_updates_per_s"] > out["value"]:
+                out["value"] = m["cell_updates_per_s"]
+                out["cores"] = m["ranks"]
+                out["sample"] = (f"{m['steps']} regular-mode steps of the {p.nx}x{p.ny} grid, {m['ranks']} MPI "
+                                 f"ranks in y-slabs ({facts['cpu_model']})")
+        else:
+            out["mpi_value"] = None
+            out["mpi_sample"] = "MPICH or oracle/_build/mpi_slab absent: y-slab threads only"
+    return out
 
 
 def cpu_model():
@@ -305,14 +389,14 @@ def bench_batch(args, K):
         phi, T = cache[it.index]
         o.set_field(0, phi)
         o.set_field(1, T)
-        cores = max(1, min(len(os.sched_getaffinity(0)), 16))
+        cores, facts = host_cores()
         tc = o.bench(2, cores)
         steps = max(2, min(2000, int(10.0 / max(tc / 2, 1e-6))))
         tc = o.bench(steps, cores)
         o.close()
         out["cpu_baseline"] = {"value": it.cells * steps / tc, "unit": "cell-updates/s", "cores": cores,
                                "kind": "port", "sample": f"{steps} regular-mode steps of instance {it.index} "
-                               f"({it.side}^2), {cores} y-slab threads ({cpu_model()})"}
+                               f"({it.side}^2), {cores} y-slab threads ({cpu_model()})", "host": facts}
     return out
 
 
@@ -430,13 +514,16 @@ def main():
     p, fields = make(args.workload, n=args.n, tile=args.tile, threads=args.threads)
     ncell = p.nx * p.ny * p.nz
 
+    chunk = graph_chunk_for(K, args.chunk)
     with Context(0) as ctx:
         ctx.set_single_launch(args.single_launch)
         inst = ctx.create(p)
         for f, a in enumerate(fields):
             inst.upload(f, a)
-        chunk = max(1, min(args.chunk, K))
-        inst.step(W, graph_chunk=chunk)
+        inst.step(W)
+        # Capture, instantiate and replay once the graph the timed region replays,
+        # whatever W is, so the timed region launches a cached, warm graph only.
+        inst.step(chunk, graph_chunk=chunk)
         ctx.sync()
         info0 = inst.info()
         u0 = info0.cell_updates
@@ -454,7 +541,19 @@ def main():
         value = active_updates / wall_s
         active_frac = active_updates / (K * ncell)
 
-        # kernel timing pass (eager, hipEvents around every step kernel)
+        # kernel time of a long graph-replayed run (the host enqueues every replay
+        # before the first ends, so no launch latency sits between the two events)
+        reps = max(1, -(-args.timing_steps // chunk))
+        u4 = inst.info().cell_updates
+        ctx.marker(2)
+        inst.step(reps * chunk, graph_chunk=chunk)
+        ctx.marker(3)
+        ctx.sync()
+        u5 = inst.info().cell_updates
+        replay_ms = ctx.elapsed_ms(2, 3)
+        replay_steps = reps * chunk
+
+        # eager pass with hipEvents around every step kernel
         TS = args.timing_steps
         ctx.timing(True)
         u2 = inst.info().cell_updates
@@ -469,10 +568,10 @@ def main():
     eager_k_s = k_ms / 1e3 / max(1, k_n)
     if single:
         # one launch per step: the kernel's duration is the GPU time per step of
-        # the timed, graph-replayed region (hipEvents at its two ends; it also
-        # holds one state-restoring launch per graph chunk and the launch gaps)
-        avg_k_s = gpu_ms / 1e3 / K
-        cells_per_launch = active_updates / K
+        # a long graph-replayed run (hipEvents at its two ends; it also holds one
+        # state-restoring launch per graph chunk and the gaps between launches)
+        avg_k_s = replay_ms / 1e3 / replay_steps
+        cells_per_launch = (u5 - u4) / replay_steps
         kernel = "single-launch step kernel (awake sets + fused stencil + detector)"
     else:
         # two launches per step: hipEvents around every step kernel in an eager pass

@@ -105,6 +105,7 @@ class Info(C.Structure):
         ("cell_updates", C.c_int64),
         ("single_launch", C.c_int32),
         ("reserved", C.c_int32),
+        ("field_updates", C.c_int64),
     ]
 
 

@@ -36,7 +36,7 @@ struct Inst {
     gu8* nflag[2];                // detector output per tile (Wakeup_Next), per set
     gu8* dF[2];                   // single-launch steps: detector bit per tile, by step parity
     gu8* dP[2];                   // ... buffer parity per tile, by step parity
-    gi32* druns;                  // ... runs per tile since the last canonicalisation
+    gi32* druns;                  // ... field-set updates per tile since the last canonicalisation
     gu64* par[2];                 // per-tile buffer parity per set: bit=1 -> buf[1] is current
     gdouble* tpart;               // conserved: per-tile change sums, [field][nt]
     gdouble* tspeed;              // KH: per-tile max wave speed, [0..nt) fresh (current U), [nt..2nt) stale

@@ -142,7 +142,8 @@ struct dba_ctx {
     int32_t* d_cols = nullptr;     // diagnostics: growth-front columns
     bool timing = false;
     int single_launch = 1;         // single-launch steps: 0 off, 1 one-instance groups, 2 also batches
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_step, ev_fin;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_step, ev_fin;  // timed intervals (events may be shared)
+    std::vector<hipEvent_t> ev_used;  // every event handed out since the timing was last reset
     std::vector<hipEvent_t> ev_pool;
     hipEvent_t markers[8] = {};
 };
@@ -618,13 +619,14 @@ void launch_finalize(const Group* g, hipStream_t s, int rotate) {
 }
 
 hipEvent_t pooled_event(dba_ctx* ctx) {
+    hipEvent_t e = nullptr;
     if (!ctx->ev_pool.empty()) {
-        hipEvent_t e = ctx->ev_pool.back();
+        e = ctx->ev_pool.back();
         ctx->ev_pool.pop_back();
-        return e;
+    } else {
+        (void)hipEventCreate(&e);
     }
-    hipEvent_t e;
-    (void)hipEventCreate(&e);
+    ctx->ev_used.push_back(e);
     return e;
 }
 
@@ -814,8 +816,7 @@ dba_status dba_finalize(dba_ctx* ctx) {
     for (auto& kv : ctx->inst) ids.push_back(kv.first);
     for (int id : ids) dba_instance_destroy(ctx, id);
     for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second.exec);
-    for (auto& e : ctx->ev_step) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
-    for (auto& e : ctx->ev_fin) (void)hipEventDestroy(e.second);
+    for (auto e : ctx->ev_used) (void)hipEventDestroy(e);
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
     for (auto m : ctx->markers) (void)hipEventDestroy(m);
     if (ctx->staging) (void)hipFree(ctx->staging);
@@ -964,7 +965,8 @@ dba_status dba_instance_info(dba_ctx* ctx, int32_t id, dba_info* out) {
     HIP_TRY(hipMemcpy(scal, in->h.scal, sizeof(scal), hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(ctr, in->h.ctr, sizeof(ctr), hipMemcpyDeviceToHost));
     out->steps_done = ctr[0];
-    out->cell_updates = ctr[1];
+    out->field_updates = ctr[1];
+    out->cell_updates = ctr[1] / std::max(1, in->h.nsets);
     out->ntiles = in->h.nt;
     out->ntiles_x = in->h.ntx;
     out->ntiles_y = in->h.nty;
@@ -1125,8 +1127,8 @@ dba_status dba_active_history(dba_ctx* ctx, int32_t id, int32_t* counts, int32_t
 dba_status dba_timing_enable(dba_ctx* ctx, int32_t enable) {
     if (!ctx) return fail(DBA_ERR_INVALID, "null ctx");
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    for (auto& e : ctx->ev_step) { ctx->ev_pool.push_back(e.first); ctx->ev_pool.push_back(e.second); }
-    for (auto& e : ctx->ev_fin) ctx->ev_pool.push_back(e.second);
+    ctx->ev_pool.insert(ctx->ev_pool.end(), ctx->ev_used.begin(), ctx->ev_used.end());
+    ctx->ev_used.clear();
     ctx->ev_step.clear();
     ctx->ev_fin.clear();
     ctx->timing = enable != 0;

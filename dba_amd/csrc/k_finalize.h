@@ -252,9 +252,15 @@ __device__ __forceinline__ void finalize_body(const Inst* __restrict__ insts, co
         }
     }
     // per-word statistics of the executed step
+    // field-set updates: set 0's fields over the tiles that ran it; the second
+    // set's field (dendrite T) over the tiles that ran either set, since a
+    // p-awake tile writes T too
     unsigned long long cells = 0ull;
-    if (rotate && valid && (ran0 | ran1)) cells = I.full_tiles ? (unsigned long long)(I.tx * I.ty * I.tz)
-                                                               : (unsigned long long)tile_cells(I, t);
+    if (rotate && valid && (ran0 | ran1)) {
+        const unsigned long long tc = I.full_tiles ? (unsigned long long)(I.tx * I.ty * I.tz)
+                                                   : (unsigned long long)tile_cells(I, t);
+        cells = tc * (unsigned long long)(nsets > 1 ? 1 + (ran0 ? 1 : 0) : 1);
+    }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) cells += __shfl_down(cells, off, 64);
     if (lane == 0) {

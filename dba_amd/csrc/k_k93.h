@@ -206,7 +206,7 @@ __global__ __launch_bounds__(NT) void k93_dense_kernel(const Inst* __restrict__ 
         I.dP[next][t] = (uint8_t)(nb.par[0][4] ^ ap);
         I.dP[next][nt + t] = (uint8_t)(nb.par[1][4] ^ at);
         if (ap | at) {
-            I.druns[t] += 1;
+            I.druns[t] += 1 + ap;  // field-set updates: T over A_p | A_T, p over A_p
             const int64_t step = I.ctr[0] + s;
             if (step < I.hist_cap) {
                 if (ap) atomicAdd((int*)&I.hist[2 * step], 1);

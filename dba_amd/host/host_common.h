@@ -131,6 +131,15 @@ inline void write_rectilinear(const std::string& name, const Grid& g,
     }
 }
 
+// Model B output_<t>.vtk: X, Y, Z, B, PHI in the reference's order
+// (write_output_vtk, example_Reactive_ModelB/kernel.cu:766-879); the layout
+// coarsening_analysis reads (it skips 20 lines and takes Z).
+inline void write_modelb_output(int t, const Grid& g, const double* x, const double* y, const double* z,
+                                const double* b, const double* phi) {
+    write_rectilinear("output_" + std::to_string(t) + ".vtk", g,
+                      {{"X", x}, {"Y", y}, {"Z", z}, {"B", b}, {"PHI", phi}});
+}
+
 // STRUCTURED_POINTS with one scalar named after the prefix: the layout
 // Compare_VTK and Extract_Tip parse (example_Dendritic_growth/CPU_version/Base.cpp:230-262).
 inline void write_structured(const std::string& prefix, int t, const Grid& g, const double* data) {

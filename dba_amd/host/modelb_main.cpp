@@ -192,9 +192,7 @@ int main() {
         o << t << " " << sx << " " << sy << " " << sz << " " << sb;
         append_line("Integral.out", "", o.str());
         std::cout << "Integral of buffer = " << sb << std::endl;
-        write_rectilinear("output_" + std::to_string(t) + ".vtk", g,
-                          {{"X", f[0].data()}, {"Y", f[1].data()}, {"Z", f[2].data()}, {"B", B.data()},
-                           {"PHI", f[3].data()}});
+        write_modelb_output(t, g, f[0].data(), f[1].data(), f[2].data(), B.data(), f[3].data());
     };
     output(0);
 
