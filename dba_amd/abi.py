@@ -139,6 +139,8 @@ SIGNATURES = [
     ("dba_set_single_launch", C.c_int32, [C.c_void_p, C.c_int32]),
     ("dba_droplets", C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                  C.POINTER(C.c_int64), C.c_void_p, C.c_int64]),
+    ("dba_droplets_noalloc", C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_double),
+                                         C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.c_void_p, C.c_int64]),
 ]
 
 _lib = None

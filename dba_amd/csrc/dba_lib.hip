@@ -30,6 +30,15 @@
 
 using namespace dba;
 
+// Device scratch of dba_droplets_noalloc: one array per GPU, held from module
+// load until the process ends. Host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device (dba_ctx::scratch); the
+// name itself is a host-side stand-in, not a device address.
+constexpr int64_t kScratchMaxCells = int64_t(1) << 22;  // 2048^2 (161^3) cells
+constexpr size_t kScratchTail = 64 * 1024;               // block partials, counters
+constexpr size_t kScratchBytes = 13 * (size_t)kScratchMaxCells + kScratchTail;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
 
 thread_local std::string g_error;
@@ -146,6 +155,7 @@ struct dba_ctx {
     std::vector<hipEvent_t> ev_used;  // every event handed out since the timing was last reset
     std::vector<hipEvent_t> ev_pool;
     hipEvent_t markers[8] = {};
+    unsigned char* scratch = nullptr;  // this device's address of the scratch array
 };
 
 namespace {
@@ -800,6 +810,15 @@ dba_status dba_init(int32_t device, dba_ctx** out) {
         delete ctx;
         return fail(DBA_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
     }
+    void* sp = nullptr;
+    e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !sp) {
+        (void)hipStreamDestroy(ctx->stream);
+        (void)hipStreamDestroy(ctx->setup);
+        delete ctx;
+        return fail(DBA_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    }
+    ctx->scratch = (unsigned char*)sp;
     HIP_TRY(hipMalloc(&ctx->d_flag, sizeof(int)));
     HIP_TRY(hipMalloc(&ctx->d_diag, sizeof(double) * (3 * kDiagBlocks + kMaxFrontCols + 8)));
     HIP_TRY(hipMalloc(&ctx->d_cols, sizeof(int32_t) * kMaxFrontCols));
@@ -1274,6 +1293,69 @@ dba_status dba_droplets(dba_ctx* ctx, int32_t id, int32_t field, double* avg, in
     for (int64_t k = 0; k < cnt[0]; k++) v[(size_t)k] = {h[(size_t)(2 * k)], h[(size_t)(2 * k + 1)]};
     std::sort(v.begin(), v.end());
     for (int64_t k = 0; k < cnt[0] && k < cap; k++) sizes[k] = v[(size_t)k].second;
+    return DBA_OK;
+}
+
+dba_status dba_droplets_noalloc(dba_ctx* ctx, int32_t id, int32_t field, double* avg, int64_t* volume,
+                                int64_t* n_droplets, int64_t* sizes, int64_t cap) {
+    if (!ctx || !avg || !volume || !n_droplets) return fail(DBA_ERR_INVALID, "null argument");
+    if (cap < 0 || (cap > 0 && !sizes)) return fail(DBA_ERR_INVALID, "sizes must hold cap >= 0 entries");
+    Instance* in = find(ctx, id);
+    if (!in) return fail(DBA_ERR_NOTFOUND, "unknown instance %d", id);
+    if (field < 0 || field >= in->h.nfields) return fail(DBA_ERR_INVALID, "field %d invalid", field);
+    const int64_t n = (int64_t)in->p.nx * in->p.ny * in->p.nz;
+    if (n > kScratchMaxCells)
+        return fail(DBA_ERR_INVALID,
+                    "dba_droplets_noalloc: %lld cells exceed the static scratch of %lld cells; use dba_droplets",
+                    (long long)n, (long long)kScratchMaxCells);
+    // carve the scratch: every region 256-byte aligned, at most kScratchBytes in all
+    unsigned char* q = ctx->scratch;
+    auto take = [&q](size_t bytes) {
+        unsigned char* r = q;
+        q += (bytes + 255) & ~(size_t)255;
+        return r;
+    };
+    uint8_t* cls = take((size_t)n);
+    int32_t* parent = (int32_t*)take(4 * (size_t)n);
+    int32_t* size = (int32_t*)take(4 * (size_t)n);
+    int32_t* key = (int32_t*)take(4 * (size_t)n);
+    double* part = (double*)take(sizeof(double) * 2 * kCcBlocks);
+    double* d_avg = (double*)take(sizeof(double));
+    int64_t* vpart = (int64_t*)take(sizeof(int64_t) * kCcBlocks);
+    int64_t* counters = (int64_t*)take(sizeof(int64_t) * 2);
+    int32_t* bcount = (int32_t*)take(sizeof(int32_t) * kCcBlocks);
+    if ((size_t)(q - ctx->scratch) > kScratchBytes)
+        return fail(DBA_ERR_INVALID, "dba_droplets_noalloc: scratch layout exceeds its array");
+    hipStream_t st = ctx->stream;
+    const dim3 g(kCcBlocks), b(kCcNT);
+    hipLaunchKernelGGL(cc_minmax_kernel, g, b, 0, st, in->d_self, 0, field, part);
+    hipLaunchKernelGGL(cc_threshold_kernel, dim3(1), dim3(64), 0, st, part, d_avg);
+    hipLaunchKernelGGL(cc_init_kernel, g, b, 0, st, in->d_self, 0, field, d_avg, cls, parent, size, key);
+    hipLaunchKernelGGL(cc_union_kernel, g, b, 0, st, cls, parent, in->p.nx, in->p.ny, in->p.nz);
+    hipLaunchKernelGGL(cc_count_kernel, g, b, 0, st, cls, parent, size, key, n, vpart);
+    // the forest is done with: parent[] becomes first[]
+    hipLaunchKernelGGL(cc_fill_kernel, g, b, 0, st, parent, n, 0, counters);
+    hipLaunchKernelGGL(cc_mark_kernel, g, b, 0, st, size, key, n, vpart, parent, counters);
+    LAUNCH_CHECK();
+    int64_t cnt[2];
+    HIP_TRY(hipMemcpyAsync(cnt, counters, sizeof(cnt), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(avg, d_avg, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    *n_droplets = cnt[0];
+    *volume = cnt[1];
+    if (cap == 0 || cnt[0] == 0) return DBA_OK;
+    // the droplets in the order of their first seeds, compacted into size[]
+    const int64_t chunk = ((n + kCcBlocks - 1) / kCcBlocks + kCcNT - 1) / kCcNT * kCcNT;
+    const int64_t k = std::min<int64_t>(cap, cnt[0]);
+    hipLaunchKernelGGL(cc_order_count_kernel, g, b, 0, st, parent, n, chunk, bcount);
+    hipLaunchKernelGGL(cc_order_write_kernel, g, b, 0, st, parent, n, chunk, bcount, size, k);
+    LAUNCH_CHECK();
+    // k int32 sizes into the front of the caller's int64 array, then widened in
+    // place from the back (entry j is read before slot j's bytes are written)
+    int32_t* narrow = (int32_t*)sizes;
+    HIP_TRY(hipMemcpyAsync(narrow, size, sizeof(int32_t) * (size_t)k, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (int64_t j = k - 1; j >= 0; j--) sizes[j] = (int64_t)narrow[j];
     return DBA_OK;
 }
 

@@ -177,4 +177,108 @@ __global__ __launch_bounds__(kCcNT) void cc_collect_kernel(const int32_t* __rest
     }
 }
 
+// ---- the same analysis with every buffer in one caller-given region ----
+// (dba_droplets_noalloc). The droplets come out in the reference's order from
+// the device itself: first[s] = size of the droplet whose first seed is cell s
+// (0 elsewhere), then an ordered compaction of first[] (contiguous chunks per
+// workgroup, chunk offsets from the per-chunk counts), so the host neither
+// allocates nor sorts.
+
+// p[0..n) = v; workgroup 0 also zeroes counters[0..2)
+__global__ __launch_bounds__(kCcNT) void cc_fill_kernel(int32_t* __restrict__ p, int64_t n, int32_t v,
+                                                        int64_t* __restrict__ counters) {
+    for (int64_t i = (int64_t)blockIdx.x * kCcNT + threadIdx.x; i < n; i += (int64_t)kCcBlocks * kCcNT) p[i] = v;
+    if (blockIdx.x == 0 && threadIdx.x < 2) counters[threadIdx.x] = 0;
+}
+
+// first[key[r]] = size[r] for every root r holding a seed (first[] zeroed);
+// counters[0] = droplets, counters[1] = volume (the seeds counted per block).
+__global__ __launch_bounds__(kCcNT) void cc_mark_kernel(const int32_t* __restrict__ size,
+                                                        const int32_t* __restrict__ key, int64_t n,
+                                                        const int64_t* __restrict__ vpart,
+                                                        int32_t* __restrict__ first,
+                                                        int64_t* __restrict__ counters) {
+    __shared__ int64_t sc[kCcNT];
+    int64_t cnt = 0;
+    for (int64_t i = (int64_t)blockIdx.x * kCcNT + threadIdx.x; i < n; i += (int64_t)kCcBlocks * kCcNT) {
+        const int32_t k = key[i], sz = size[i];
+        if (k == INT32_MAX || sz == 0) continue;
+        first[k] = sz;
+        cnt++;
+    }
+    sc[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int s = kCcNT / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sc[threadIdx.x] += sc[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && sc[0] > 0) atomicAdd((unsigned long long*)&counters[0], (unsigned long long)sc[0]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        int64_t v = 0;
+        for (int b = 0; b < kCcBlocks; b++) v += vpart[b];
+        counters[1] = v;
+    }
+}
+
+// Chunk of workgroup b: cells [b * chunk, min(n, (b + 1) * chunk)).
+__device__ __forceinline__ void cc_chunk(int64_t n, int64_t chunk, int64_t* lo, int64_t* hi) {
+    *lo = (int64_t)blockIdx.x * chunk;
+    *hi = *lo + chunk < n ? *lo + chunk : n;
+}
+
+// bcount[b] = droplet starts (first[] != 0) in chunk b
+__global__ __launch_bounds__(kCcNT) void cc_order_count_kernel(const int32_t* __restrict__ first, int64_t n,
+                                                               int64_t chunk, int32_t* __restrict__ bcount) {
+    __shared__ int32_t sc[kCcNT];
+    int64_t lo, hi;
+    cc_chunk(n, chunk, &lo, &hi);
+    int32_t c = 0;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kCcNT) c += first[i] != 0;
+    sc[threadIdx.x] = c;
+    __syncthreads();
+    for (int s = kCcNT / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sc[threadIdx.x] += sc[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bcount[blockIdx.x] = sc[0];
+}
+
+// out[k] = the k-th droplet start's size in cell order, for k < cap
+__global__ __launch_bounds__(kCcNT) void cc_order_write_kernel(const int32_t* __restrict__ first, int64_t n,
+                                                               int64_t chunk, const int32_t* __restrict__ bcount,
+                                                               int32_t* __restrict__ out, int64_t cap) {
+    constexpr int NW = kCcNT / 64;
+    __shared__ int64_t sb[kCcNT];
+    __shared__ int32_t sw[NW];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    int64_t lo, hi;
+    cc_chunk(n, chunk, &lo, &hi);
+    // this chunk's offset: the starts of every earlier chunk
+    int64_t b = 0;
+    for (int j = tid; j < (int)blockIdx.x; j += kCcNT) b += bcount[j];
+    sb[tid] = b;
+    __syncthreads();
+    for (int s = kCcNT / 2; s > 0; s >>= 1) {
+        if (tid < s) sb[tid] += sb[tid + s];
+        __syncthreads();
+    }
+    int64_t base = sb[0];
+    for (int64_t w0 = lo; w0 < hi; w0 += kCcNT) {
+        const int64_t i = w0 + tid;
+        const int32_t v = i < hi ? first[i] : 0;
+        const uint64_t m = __ballot(v != 0);
+        if (lane == 0) sw[wid] = __popcll(m);
+        __syncthreads();
+        int64_t pre = 0, tot = 0;
+        for (int w = 0; w < NW; w++) {
+            pre += w < wid ? sw[w] : 0;
+            tot += sw[w];
+        }
+        const int64_t k = base + pre + __popcll(m & ((1ull << lane) - 1ull));
+        if (v != 0 && k < cap) out[k] = v;
+        base += tot;
+        __syncthreads();
+    }
+}
+
 }  // namespace dba

@@ -90,14 +90,16 @@ class Instance:
         abi.check(self.ctx.lib.dba_kh_vy_mode(self.ctx.h, self.id, C.byref(out)))
         return out.value
 
-    def droplets(self, field, cap=1 << 20):
+    def droplets(self, field, cap=1 << 20, noalloc=False):
         """Droplet analysis on the device: (avg, volume, sizes in the reference's
         order, mean radius). The radius follows coarsening_analysis: cbrt(3 V /
-        (4 pi N)) in 3D, sqrt(V / (pi N)) in 2D (nan without droplets)."""
+        (4 pi N)) in 3D, sqrt(V / (pi N)) in 2D (nan without droplets).
+        noalloc: the entry point that works in the library's static device
+        scratch (grids of at most 2^22 cells) instead of allocating."""
         avg, vol, cnt = C.c_double(), C.c_int64(), C.c_int64()
         sizes = np.zeros(max(cap, 1), dtype=np.int64)
-        abi.check(self.ctx.lib.dba_droplets(self.ctx.h, self.id, field, C.byref(avg), C.byref(vol), C.byref(cnt),
-                                            sizes.ctypes.data, cap))
+        fn = self.ctx.lib.dba_droplets_noalloc if noalloc else self.ctx.lib.dba_droplets
+        abi.check(fn(self.ctx.h, self.id, field, C.byref(avg), C.byref(vol), C.byref(cnt), sizes.ctypes.data, cap))
         n = cnt.value
         sizes = sizes[:min(n, cap)]
         if n == 0:
