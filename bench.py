@@ -10,7 +10,7 @@ Workloads (--workload):
   kh        compressible Euler Kelvin-Helmholtz 2048^2 FP64, gamma 1.4, CFL 0.4,
             reflective y walls, 16x16 tiles, density-gradient wake-up (eps 0.1).
   batch     64 independent 2D Kobayashi dendrites, sides log-uniform in [256, 2048]
-            (rounded to multiples of 32), per-instance delta, j, K and 1-3 seeds
+            (to the nearest cell), per-instance delta, j, K and 1-3 seeds
             (dba_amd.batch), 5000 steps, one merged-worklist group per step;
             tile and workgroup size from the autotuner. Reports the batched
             rate next to the plain one-instance-at-a-time loop (16x16 tiles).

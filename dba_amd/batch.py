@@ -10,8 +10,9 @@ one-simulation-per-process loop (example_Dendritic_growth/kernel.cu:913-1011);
 `graph` replays captured chunks of steps.
 
 The sweep (64 instances by default):
-  side      log-uniform in [256, 2048], rounded to a multiple of 32 (so tiles of
-            8, 16 and 32 all divide it)
+  side      log-uniform in [256, 2048], rounded to the nearest cell; a side one
+            more than a multiple of 8 becomes one cell longer (a partial last
+            tile must be two cells or more for tiles of 8, 16 and 32)
   delta     uniform in [0.01, 0.05]
   j         4 or 6 with equal probability
   K         uniform in [1.2, 2.0]
@@ -57,14 +58,19 @@ class SweepItem:
         return self.side * self.side
 
 
-def log_uniform_side(u, lo, hi, multiple):
-    """exp(log lo + u (log hi - log lo)) rounded to the nearest multiple, kept in [lo, hi]."""
+def log_uniform_side(u, lo, hi, multiple=1):
+    """exp(log lo + u (log hi - log lo)) rounded to the nearest multiple, kept in
+    [lo, hi]; a side of 8 k + 1 cells becomes 8 k + 2 (the library refuses a
+    partial last tile one cell wide, and 8 | 16 | 32)."""
     s = math.exp(math.log(lo) + u * (math.log(hi) - math.log(lo)))
     s = int(round(s / multiple)) * multiple
-    return max(lo, min(hi, s))
+    s = max(lo, min(hi, s))
+    if s % 8 == 1:
+        s += 1
+    return s
 
 
-def make_sweep(n=64, master_seed=2024, side_lo=256, side_hi=2048, multiple=32):
+def make_sweep(n=64, master_seed=2024, side_lo=256, side_hi=2048, multiple=1):
     items = []
     for i in range(n):
         # one stream per instance: side, delta, j, K, seed count, 3 x (cx, cy)

@@ -50,8 +50,8 @@ def base_params(model, nx, ny, nz=1, tile=16, mode=abi.MODE_DBA):
 
 
 def dendrite_k93_params(n=512, tile=16, mode=abi.MODE_DBA, delta=0.02, aniso_j=4, latent_k=1.6,
-                        theta0=0.0, threads=0, history_cap=0, bc=abi.BC_NOFLUX):
-    p = base_params(abi.MODEL_DENDRITE_K93, n, n, 1, tile, mode)
+                        theta0=0.0, threads=0, history_cap=0, bc=abi.BC_NOFLUX, ny=None):
+    p = base_params(abi.MODEL_DENDRITE_K93, n, ny or n, 1, tile, mode)
     p.bc_x = p.bc_y = bc
     p.dx = p.dy = p.dz = 0.03
     p.dt = 1e-4

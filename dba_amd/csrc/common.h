@@ -226,6 +226,21 @@ __device__ __forceinline__ void nb_fill(NbTable& nb, CInst& I, int t, int x0, in
         nb.shift[f][d] = f < ncons ? (I.scal[8 + f] - I.s_tile[((int64_t)f * 2 + nb.par[0][d]) * I.nt + tt]) : 0.0;
 }
 
+// Direction (0 lower, 1 the tile itself, 2 upper) along one axis of the owner of
+// the mapped cell m, for tile i of n tiles of TB cells: the tile holding m is the
+// tile, its upper neighbour (after the boundary map) or else its lower one. Used
+// for tiles whose halo can reach past a partial last tile, where the owner no
+// longer follows from the cell's offset in the tile's region.
+__device__ __forceinline__ int owner_dir1(int m, int TB, int i, int n, int bc) {
+    const int o = m / TB;
+    if (o == i) return 1;
+    return o == nb_step1(i, 1, n, bc) ? 2 : 0;
+}
+
+// Cells along one axis of tile i of tiles of TB cells on an axis of n cells:
+// TB, except for a partial last tile.
+__device__ __forceinline__ int tile_extent(int i, int TB, int n) { return min(TB, n - i * TB); }
+
 // Direction index of a cell at local coordinate l (0..W-1) of a region whose
 // interior [h, h+TB) is the tile.
 template <int TB, int H>

@@ -188,8 +188,11 @@ dba_status validate(const dba_params& p) {
         if (p.nz != 1) return fail(DBA_ERR_INVALID, "model %d is 2D (nz must be 1)", p.model);
         if (p.tile_x != p.tile_y || !is_pow2_tile(p.tile_x))
             return fail(DBA_ERR_INVALID, "tile must be square 8, 16 or 32 (got %dx%d)", p.tile_x, p.tile_y);
-        if (p.nx % p.tile_x || p.ny % p.tile_y)
-            return fail(DBA_ERR_INVALID, "grid %dx%d must be a multiple of the tile %d", p.nx, p.ny, p.tile_x);
+        // a partial last tile needs two cells or more: its neighbours' halos and its
+        // own ring stay within the 3x3 tiles around each tile
+        if (p.nx % p.tile_x == 1 || p.ny % p.tile_y == 1)
+            return fail(DBA_ERR_INVALID, "grid %dx%d leaves a partial tile one cell wide (tile %d)", p.nx, p.ny,
+                        p.tile_x);
         if (p.nx < p.tile_x * 1 || p.ny < p.tile_y) return fail(DBA_ERR_INVALID, "grid smaller than a tile");
     }
     if (p.model == DBA_MODEL_KH) {

@@ -16,28 +16,46 @@
 //      cells of sleeping neighbours use the mu they stored when they last ran,
 //      which is what the reference's full mu array holds for them;
 //   3. c' = c + dt (M lap mu - Gamma (c - cbar)) into the other buffer;
-//   4. detector any(|lap c| > eps) -> one byte per tile; the tile's own
-//      edge-ring mu is stored for its neighbours.
-// HBM traffic per cell: read c, write c' (16 B) + the mu ring (4(T-1)/T^2 of cells),
-// kept as one contiguous record of 4 TB - 4 values per tile (ring order of
-// TileRing) so that its column cells do not each cost a partial-line write.
+//   4. detector any(|lap c| > eps) -> one byte per tile; a tile that did not
+//      fire stores its own edge-ring mu for its neighbours (one that fired is
+//      awake next step, so nobody reads that step's ring).
+// HBM traffic per cell: read c, write c' (16 B) + the mu ring (4(T-1)/T^2 of cells)
+// of tiles that did not fire, kept as one contiguous record of 4 TB - 4 values
+// per tile (ring order of TileRing) so that its column cells do not each cost a
+// partial-line write.
 #pragma once
 #include "common.h"
 #include "k_finalize.h"
 
 namespace dba {
 
-// Ring cells of a TB x TB tile, in record order: bottom row, top row, left
-// column, right column (the columns without their corners).
+// Ring cells of a w x h tile (TB x TB, or smaller for a partial last tile), in
+// record order: bottom row, top row, left column, right column (the columns
+// without their corners). A tile's record holds N = 4 TB - 4 slots; a partial
+// tile uses the first 2 w + 2 h - 4 of them.
 template <int TB>
 struct TileRing {
     static constexpr int N = 4 * TB - 4;
-    __device__ __forceinline__ static int slot(int lx, int ly) {
+    __device__ __forceinline__ static int slot(int lx, int ly, int w = TB, int h = TB) {
         if (ly == 0) return lx;
-        if (ly == TB - 1) return TB + lx;
-        if (lx == 0) return 2 * TB + ly - 1;
-        if (lx == TB - 1) return 3 * TB - 2 + ly - 1;
+        if (ly == h - 1) return w + lx;
+        if (lx == 0) return 2 * w + ly - 1;
+        if (lx == w - 1) return 2 * w + h - 2 + ly - 1;
         return -1;
+    }
+    // the cell of slot k (the inverse of slot)
+    __device__ __forceinline__ static void cell(int k, int* lx, int* ly, int w = TB, int h = TB) {
+        if (k < w) { *lx = k; *ly = 0; }
+        else if (k < 2 * w) { *lx = k - w; *ly = h - 1; }
+        else if (k < 2 * w + h - 2) { *lx = 0; *ly = k - 2 * w + 1; }
+        else { *lx = w - 1; *ly = k - (2 * w + h - 2) + 1; }
+    }
+    __device__ __forceinline__ static int size(int w = TB, int h = TB) { return 2 * w + 2 * h - 4; }
+    // the stored ring value of cell (mx, my) of its sleeping owner tile
+    __device__ __forceinline__ static double stale(const gdouble* store, int owner, int mx, int my, int nx, int ny) {
+        const int ox = mx & ~(TB - 1), oy = my & ~(TB - 1);
+        const int sl = slot(mx - ox, my - oy, min(TB, nx - ox), min(TB, ny - oy));
+        return sl >= 0 ? store[(int64_t)owner * N + sl] : 0.0;
     }
 };
 
@@ -66,6 +84,100 @@ __device__ __forceinline__ void strip_lap(const double* s, int cx, int cy0, doub
     }
 }
 
+// One partial tile's step (the last tile column / row when the grid is not a
+// multiple of the tile): w x h cells, a generic cell loop instead of the
+// register strips, and the owner of every halo cell found from its mapped
+// position (past a partial tile, the offset in the tile's region no longer
+// tells which neighbour holds a cell). The same arithmetic as ch_tile.
+template <int TB, int NT, bool NINE, bool WORKLIST>
+__device__ __noinline__ int ch_tile_partial(CInst& I, int t, uint64_t rec, NbTable& nb, double* sc, double* smu,
+                                            double* sred, int* sflag) {
+    constexpr int W2 = TB + 4;
+    constexpr int W1 = TB + 2;
+    const int tid = threadIdx.x;
+    const int nx = I.nx, ny = I.ny, ntx = I.ntx, nty = I.nty, bcx = I.bc_x, bcy = I.bc_y;
+    const int bcyo = bcy == 0 ? 0 : 1;
+    const int ti = t % ntx, tj = t / ntx;
+    const int x0 = ti * TB, y0 = tj * TB;
+    const int w = tile_extent(ti, TB, nx), h = tile_extent(tj, TB, ny);
+    const bool regular = I.mode == 0;
+    const bool cons = I.mode == 2;
+    const gdouble* __restrict__ b0 = I.buf[0][0];
+    const gdouble* __restrict__ b1 = I.buf[1][0];
+    gdouble* __restrict__ mu_store = I.aux[0];
+    const double kappa = I.kappa, eps = I.thresh_a;
+    const double inv = NINE ? I.inv6dx2 : I.invdx2;
+    if (WORKLIST) {
+        nb_fill<TB>(nb, I, t, x0, y0, regular, 1, cons ? 1 : 0, rec);
+        __syncthreads();
+    }
+    const int pself = nb.par[0][4];
+    gdouble* __restrict__ nxt = pself ? I.buf[0][0] : I.buf[1][0];
+    // 1. c on the tile and a 2-cell halo
+    for (int k = tid; k < (w + 4) * (h + 4); k += NT) {
+        const int lx = k % (w + 4), ly = k / (w + 4);
+        const int mx = map_index(x0 + lx - 2, nx, bcx), my = map_index(y0 + ly - 2, ny, bcy);
+        const int d = owner_dir1(mx, TB, ti, ntx, bcx) + 3 * owner_dir1(my, TB, tj, nty, bcyo);
+        sc[lx + ly * W2] = (nb.par[0][d] ? b1 : b0)[mx + (int64_t)my * nx] - nb.shift[0][d];
+    }
+    __syncthreads();
+    // 2. mu on the tile and a 1-cell ring (stored ring values of sleeping owners)
+    int flag = 0;
+    for (int k = tid; k < (w + 2) * (h + 2); k += NT) {
+        const int lx = k % (w + 2), ly = k / (w + 2);
+        const bool own = lx >= 1 && lx <= w && ly >= 1 && ly <= h;
+        const int mx = map_index(x0 + lx - 1, nx, bcx), my = map_index(y0 + ly - 1, ny, bcy);
+        const int d = own ? 4 : owner_dir1(mx, TB, ti, ntx, bcx) + 3 * owner_dir1(my, TB, tj, nty, bcyo);
+        double mu;
+        if (own || nb.act[0][d]) {
+            const double lcv = NINE ? lds_lap9<W2>(sc, lx + 1, ly + 1, inv) : lds_lap5<W2>(sc, lx + 1, ly + 1, inv);
+            const double cv = sc[(lx + 1) + (ly + 1) * W2];
+            mu = (cv * cv * cv - cv) - kappa * lcv;
+            if (own) flag |= fabs(lcv) > eps;
+        } else {
+            mu = TileRing<TB>::stale(mu_store, nb.tile[d], mx, my, nx, ny);
+        }
+        smu[lx + ly * W1] = mu;
+    }
+    __syncthreads();
+    // 3. c' on the tile
+    double part = 0.0;
+    {
+        const double dt = I.dt, mob = I.mobility, rg = I.react_gamma, cbar = I.cbar;
+        for (int k = tid; k < w * h; k += NT) {
+            const int lx = k % w, ly = k / w;
+            const double lm = NINE ? lds_lap9<W1>(smu, lx + 1, ly + 1, inv) : lds_lap5<W1>(smu, lx + 1, ly + 1, inv);
+            const double cv = sc[(lx + 2) + (ly + 2) * W2];
+            const double change = dt * (mob * lm - rg * (cv - cbar));
+            nxt[(x0 + lx) + (int64_t)(y0 + ly) * nx] = cv + change;
+            part += change;
+        }
+    }
+    // 4. detector bit; the ring for the neighbours when the tile did not fire
+    const int any = regular ? 0 : block_any<NT>(flag, sflag);
+    if (WORKLIST && !regular && tid == 0) I.nflag[0][t] = (uint8_t)any;
+    if (!regular && !any) {
+        for (int k = tid; k < TileRing<TB>::size(w, h); k += NT) {
+            int lx, ly;
+            TileRing<TB>::cell(k, &lx, &ly, w, h);
+            mu_store[(int64_t)t * TileRing<TB>::N + k] = smu[(lx + 1) + (ly + 1) * W1];
+        }
+    }
+    if (cons) {
+        const double tot = block_sum<NT>(part, sred);
+        if (tid == 0) {
+            if (WORKLIST) {
+                I.tpart[t] = tot;
+                I.s_tile[(pself ^ 1) * I.nt + t] = I.scal[8];
+            } else {
+                nb.tot = tot;
+            }
+        }
+    }
+    __syncthreads();
+    return any;
+}
+
 // One tile's step. WORKLIST: fill the neighbour table from the record and store
 // the detector byte here; otherwise the caller has filled the table (then a
 // barrier) and stores the returned detector bit any(|lap c| > eps) (0 in regular
@@ -82,6 +194,8 @@ __device__ __forceinline__ int ch_tile(CInst& I, int t, uint64_t rec, NbTable& n
     const int tx = tid % TB, ty = tid / TB;
     const int nx = I.nx, ny = I.ny, ntx = I.ntx, bcx = I.bc_x, bcy = I.bc_y;
     const int x0 = (t % ntx) * TB, y0 = (t / ntx) * TB;
+    if (!I.full_tiles && (x0 + TB > nx || y0 + TB > ny))
+        return ch_tile_partial<TB, NT, NINE, WORKLIST>(I, t, rec, nb, sc, smu, sred, sflag);
     const bool regular = I.mode == 0;
     const bool cons = I.mode == 2;
     const gdouble* __restrict__ b0 = I.buf[0][0];
@@ -129,8 +243,6 @@ __device__ __forceinline__ int ch_tile(CInst& I, int t, uint64_t rec, NbTable& n
             flag |= fabs(lc[r]) > eps;
             const int ly = ty * R + r;
             smu[(tx + 1) + (ly + 1) * W1] = mu;
-            const int slot = TileRing<TB>::slot(tx, ly);
-            if (!regular && slot >= 0) mu_store[(int64_t)t * TileRing<TB>::N + slot] = mu;
         }
     }
     for (int k = tid; k < NH1; k += NT) {
@@ -148,8 +260,7 @@ __device__ __forceinline__ int ch_tile(CInst& I, int t, uint64_t rec, NbTable& n
             // the ring value the sleeping owner stored (a ring cell of it: the map
             // only lands inside the running tile itself)
             const int mx = map_index(x0 + lx - 1, nx, bcx), my = map_index(y0 + ly - 1, ny, bcy);
-            const int sl = TileRing<TB>::slot(mx & (TB - 1), my & (TB - 1));
-            mu = sl >= 0 ? mu_store[(int64_t)nb.tile[d] * TileRing<TB>::N + sl] : 0.0;
+            mu = TileRing<TB>::stale(mu_store, nb.tile[d], mx, my, nx, ny);
         }
         smu[lx + ly * W1] = mu;
     }
@@ -169,9 +280,19 @@ __device__ __forceinline__ int ch_tile(CInst& I, int t, uint64_t rec, NbTable& n
         }
     }
 
-    // 4. detector bit, conserved bookkeeping
+    // 4. detector bit; the edge-ring mu for the neighbours, only when the tile
+    //    did not fire: a tile that fired is awake next step (the dilation keeps
+    //    the tile itself), and a ring is read only while its owner sleeps, so
+    //    the ring a sleeping owner presents is the one of the last step it ran
     const int any = regular ? 0 : block_any<NT>(flag, sflag);
     if (WORKLIST && !regular && tid == 0) I.nflag[0][t] = (uint8_t)any;
+    if (!regular && !any) {
+        for (int k = tid; k < TileRing<TB>::N; k += NT) {
+            int lx, ly;
+            TileRing<TB>::cell(k, &lx, &ly);
+            mu_store[(int64_t)t * TileRing<TB>::N + k] = smu[(lx + 1) + (ly + 1) * W1];
+        }
+    }
     if (cons) {
         const double tot = block_sum<NT>(part, sred);
         if (tid == 0) {

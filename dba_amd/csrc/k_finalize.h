@@ -528,7 +528,9 @@ __device__ __forceinline__ void dense_canon_seg(CInst& I, int t0, int ntile, int
         }
     }
     if (valid) I.druns[t] = 0;
-    unsigned long long cells = valid ? (unsigned long long)r * (unsigned long long)(I.tx * I.ty * I.tz) : 0ull;
+    unsigned long long cells =
+        valid ? (unsigned long long)r * (unsigned long long)(I.full_tiles ? I.tx * I.ty * I.tz : tile_cells(I, t))
+              : 0ull;
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) cells += __shfl_down(cells, off, 64);
     if (lane == 0) {

@@ -58,7 +58,9 @@ struct K93Lds {
 // record and store the detector bytes here; otherwise the caller has filled the
 // table and stores the returned bits (p) | (T) << 1 (uniform over the
 // workgroup, 0 in regular mode; bit s is meaningful only for a set that ran).
-template <int TB, int NT, bool WORKLIST>
+// PARTIAL: a partial last tile (w x h cells) when the grid is not a multiple of
+// the tile; the owner of each halo cell then comes from its mapped position.
+template <int TB, int NT, bool WORKLIST, bool PARTIAL = false>
 __device__ __forceinline__ int k93_tile(CInst& I, int t, uint64_t rec, NbTable& nb, K93Lds<TB>& L,
                                         int* sflag) {
     constexpr int W2 = TB + 4;
@@ -69,8 +71,15 @@ __device__ __forceinline__ int k93_tile(CInst& I, int t, uint64_t rec, NbTable& 
     double* sFy = L.sFy;
     double* sE2 = L.sE2;
     const int tid = threadIdx.x;
-    const int nx = I.nx, ny = I.ny, ntx = I.ntx, bcx = I.bc_x, bcy = I.bc_y;
-    const int x0 = (t % ntx) * TB, y0 = (t / ntx) * TB;
+    const int nx = I.nx, ny = I.ny, ntx = I.ntx, nty = I.nty, bcx = I.bc_x, bcy = I.bc_y;
+    const int ti = t % ntx, tj = t / ntx;
+    const int x0 = ti * TB, y0 = tj * TB;
+    // tile extent (TB x TB unless partial) and the owner direction of a mapped cell
+    const int w = PARTIAL ? tile_extent(ti, TB, nx) : TB, h = PARTIAL ? tile_extent(tj, TB, ny) : TB;
+    const int bcyo = bcy == 0 ? 0 : 1;
+    auto odir = [&](int mx, int my) {
+        return owner_dir1(mx, TB, ti, ntx, bcx) + 3 * owner_dir1(my, TB, tj, nty, bcyo);
+    };
     const bool regular = I.mode == 0;
     const gdouble* __restrict__ p0 = I.buf[0][0];
     const gdouble* __restrict__ p1 = I.buf[1][0];
@@ -87,35 +96,37 @@ __device__ __forceinline__ int k93_tile(CInst& I, int t, uint64_t rec, NbTable& 
 
     // stage phi (2-cell halo) when p-awake, T (1-cell halo) when T-awake,
     // and phi / T of the tile itself always (the T update needs phi' - phi)
-    for (int k = tid; k < W2 * W2; k += NT) {
-        const int lx = k % W2, ly = k / W2;
-        const int d = nb_dir1<TB, 2>(lx) + 3 * nb_dir1<TB, 2>(ly);
-        if (d != 4 && !ap) continue;
+    for (int k = tid; k < (w + 4) * (h + 4); k += NT) {
+        const int lx = k % (w + 4), ly = k / (w + 4);
+        const bool own = lx >= 2 && lx < w + 2 && ly >= 2 && ly < h + 2;
+        if (!own && !ap) continue;
         const int mx = map_index(x0 + lx - 2, nx, bcx), my = map_index(y0 + ly - 2, ny, bcy);
-        sp[k] = (nb.par[0][d] ? p1 : p0)[mx + (int64_t)my * nx];
+        const int d = PARTIAL ? (own ? 4 : odir(mx, my)) : nb_dir1<TB, 2>(lx) + 3 * nb_dir1<TB, 2>(ly);
+        sp[lx + ly * W2] = (nb.par[0][d] ? p1 : p0)[mx + (int64_t)my * nx];
     }
-    for (int k = tid; k < W1 * W1; k += NT) {
-        const int lx = k % W1, ly = k / W1;
-        const int d = nb_dir1<TB, 1>(lx) + 3 * nb_dir1<TB, 1>(ly);
-        if (d != 4 && !at) continue;
+    for (int k = tid; k < (w + 2) * (h + 2); k += NT) {
+        const int lx = k % (w + 2), ly = k / (w + 2);
+        const bool own = lx >= 1 && lx <= w && ly >= 1 && ly <= h;
+        if (!own && !at) continue;
         const int mx = map_index(x0 + lx - 1, nx, bcx), my = map_index(y0 + ly - 1, ny, bcy);
-        sT[k] = (nb.par[1][d] ? T1 : T0)[mx + (int64_t)my * nx];
+        const int d = PARTIAL ? (own ? 4 : odir(mx, my)) : nb_dir1<TB, 1>(lx) + 3 * nb_dir1<TB, 1>(ly);
+        sT[lx + ly * W1] = (nb.par[1][d] ? T1 : T0)[mx + (int64_t)my * nx];
     }
     __syncthreads();
 
     // eps eps' grad phi on the tile + ring, eps^2 on the tile
     if (ap) {
         const double inv2dx = I.inv2dx;
-        for (int k = tid; k < W1 * W1; k += NT) {
-            const int lx = k % W1, ly = k / W1;
+        for (int k = tid; k < (w + 2) * (h + 2); k += NT) {
+            const int lx = k % (w + 2), ly = k / (w + 2);
             const int cx = lx + 1, cy = ly + 1;
             const double gx = (sp[(cx + 1) + cy * W2] - sp[(cx - 1) + cy * W2]) * inv2dx;
             const double gy = (sp[cx + (cy + 1) * W2] - sp[cx + (cy - 1) * W2]) * inv2dx;
             double e, a;
             k93_aniso(I, gx, gy, &e, &a);
-            sFx[k] = a * gx;
-            sFy[k] = a * gy;
-            if (lx >= 1 && lx <= TB && ly >= 1 && ly <= TB) sE2[(lx - 1) + (ly - 1) * TB] = e * e;
+            sFx[lx + ly * W1] = a * gx;
+            sFy[lx + ly * W1] = a * gy;
+            if (lx >= 1 && lx <= w && ly >= 1 && ly <= h) sE2[(lx - 1) + (ly - 1) * TB] = e * e;
         }
     }
     __syncthreads();
@@ -124,8 +135,8 @@ __device__ __forceinline__ int k93_tile(CInst& I, int t, uint64_t rec, NbTable& 
     const double inv2dx = I.inv2dx, invdx2 = I.invdx2, alpha_pi = I.alpha_pi, gam = I.gamma;
     const double t_eq = I.t_eq, dt_tau = I.dt_tau, dt = I.dt, lk = I.latent_k;
     const double mp = I.thresh_a, mT = I.thresh_b;
-    for (int k = tid; k < TB * TB; k += NT) {
-        const int lx = k % TB, ly = k / TB;
+    for (int k = tid; k < w * h; k += NT) {
+        const int lx = k % w, ly = k / w;
         const int64_t gi = (x0 + lx) + (int64_t)(y0 + ly) * nx;
         const double ph = sp[(lx + 2) + (ly + 2) * W2];
         const double Tv = sT[(lx + 1) + (ly + 1) * W1];
@@ -136,7 +147,7 @@ __device__ __forceinline__ int k93_tile(CInst& I, int t, uint64_t rec, NbTable& 
                                   (sFy[(fx + 1) + fy * W1] - sFy[(fx - 1) + fy * W1])) * inv2dx;
             const double lap = lds_lap5<W2>(sp, cx, cy, invdx2);
             const double m = alpha_pi * atan(gam * (t_eq - Tv));
-            const double rhs = (cross + sE2[k] * lap) + (ph * (1.0 - ph)) * ((ph - 0.5) + m);
+            const double rhs = (cross + sE2[lx + ly * TB] * lap) + (ph * (1.0 - ph)) * ((ph - 0.5) + m);
             pn = ph + dt_tau * rhs;
             pnew[gi] = pn;
             fp |= (pn >= mp && pn <= (1.0 - mp));
@@ -163,6 +174,12 @@ __device__ __forceinline__ int k93_tile(CInst& I, int t, uint64_t rec, NbTable& 
     return bits;
 }
 
+// A partial last tile of an instance whose grid is not a multiple of the tile.
+template <int TB>
+__device__ __forceinline__ bool k93_partial(CInst& I, int t) {
+    return !I.full_tiles && ((t % I.ntx) * TB + TB > I.nx || (t / I.ntx) * TB + TB > I.ny);
+}
+
 template <int TB, int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k93_step_kernel(const WorkList wl) {
     __shared__ K93Lds<TB> L;
@@ -177,7 +194,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k9
         int t;
         uint64_t rec;
         if (!work_next(wl, cur, total, &Ip, &t, &rec)) break;
-        k93_tile<TB, NT, true>(*Ip, t, rec, nb, L, sflag);
+        if (k93_partial<TB>(*Ip, t)) k93_tile<TB, NT, true, true>(*Ip, t, rec, nb, L, sflag);
+        else k93_tile<TB, NT, true>(*Ip, t, rec, nb, L, sflag);
     }
 }
 
@@ -199,7 +217,9 @@ __global__ __launch_bounds__(NT) void k93_dense_kernel(const Inst* __restrict__ 
     __syncthreads();
     const int ap = nb.act[0][4], at = nb.act[1][4];
     int bits = 0;
-    if (ap | at) bits = k93_tile<TB, NT, false>(I, t, 0, nb, L, sflag);
+    if (ap | at)
+        bits = k93_partial<TB>(I, t) ? k93_tile<TB, NT, false, true>(I, t, 0, nb, L, sflag)
+                                     : k93_tile<TB, NT, false>(I, t, 0, nb, L, sflag);
     if (threadIdx.x == 0) {
         I.dF[next][t] = (uint8_t)(ap ? bits & 1 : 0);
         I.dF[next][nt + t] = (uint8_t)(at ? bits >> 1 : 0);

@@ -92,6 +92,13 @@ struct KhEdge {
         if (lx == TB - 1) return 3 * TB - 2 + ly - 1;
         return -1;
     }
+    // the cell of ring slot k (the inverse of ring)
+    __device__ __forceinline__ static void cell(int k, int* lx, int* ly) {
+        if (k < TB) { *lx = k; *ly = 0; }
+        else if (k < 2 * TB) { *lx = k - TB; *ly = TB - 1; }
+        else if (k < 3 * TB - 2) { *lx = 0; *ly = k - 2 * TB + 1; }
+        else { *lx = TB - 1; *ly = k - (3 * TB - 2) + 1; }
+    }
 };
 
 template <int TB, int NT>
@@ -183,20 +190,13 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
         }
         __syncthreads();
 
-        // stale speed (primitives at the start of the step) + the tile's ring primitives
+        // stale speed (primitives at the start of the step)
         double vstale = 0.0;
-        for (int k = tid; k < TB * TB; k += NT) {
-            const int lx = k % TB, ly = k / TB;
-            const int q = (lx + 2) + (ly + 2) * W2;
-            const double r = sP[q], u = sP[A2 + q], v = sP[2 * A2 + q], p = sP[3 * A2 + q];
-            if (cfl) vstale = fmax(vstale, kh_speed(c, r, u, v, p));
-            const int ri = E::ring(lx, ly);
-            if (dba && ri >= 0) {
-                gdouble* e = I.aux[0] + (int64_t)t * E::R;
-                e[ri] = r;
-                e[E::RING + ri] = u;
-                e[2 * E::RING + ri] = v;
-                e[3 * E::RING + ri] = p;
+        if (cfl) {
+            for (int k = tid; k < TB * TB; k += NT) {
+                const int lx = k % TB, ly = k / TB;
+                const int q = (lx + 2) + (ly + 2) * W2;
+                vstale = fmax(vstale, kh_speed(c, sP[q], sP[A2 + q], sP[2 * A2 + q], sP[3 * A2 + q]));
             }
         }
 
@@ -254,30 +254,12 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
                     ? fabs(r_x) + fabs(r_y)
                     : fabs(r_x) + fabs(r_y) + fabs(p_x) + fabs(p_y) + fabs(u_x) + fabs(u_y) + fabs(v_x) + fabs(v_y);
                 flag |= metric > I.thresh_a;
-                if (dba && (lx == 1 || ly == 1)) {
-                    gdouble* e = I.aux[0] + (int64_t)t * E::R;
-                    const double pr[4] = {rp, up, vp, pp};
-                    if (lx == 1) {
-                        const double gx[4] = {r_x, u_x, v_x, p_x};
-                        for (int q = 0; q < 4; q++) {
-                            e[E::LEFT + q * TB + ly - 1] = pr[q];
-                            e[E::LEFT + (4 + q) * TB + ly - 1] = gx[q];
-                        }
-                    }
-                    if (ly == 1) {
-                        const double gy[4] = {r_y, u_y, v_y, p_y};
-                        for (int q = 0; q < 4; q++) {
-                            e[E::BOTTOM + q * TB + lx - 1] = pr[q];
-                            e[E::BOTTOM + (4 + q) * TB + lx - 1] = gy[q];
-                        }
-                    }
-                }
             }
         }
         __syncthreads();
 
         // Grad_x / Grad_y of W1 cell (lx, ly) recomputed from the primitives
-        // (example_Kelvin_Helmholtz/KH_DBA.cu:249-275, as in phase 2)
+        // (example_Kelvin_Helmholtz/KH_DBA.cu:249-275, the expressions of phase 2)
         auto grad_x = [&](int q, int lx, int ly) {
             const int p0 = (lx + 1) + (ly + 1) * W2;
             return (sP[q * A2 + p0 + 1] - sP[q * A2 + p0 - 1]) / 2.0 / c.dx;
@@ -286,6 +268,34 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
             const int p0 = (lx + 1) + (ly + 1) * W2;
             return (sP[q * A2 + p0 + W2] - sP[q * A2 + p0 - W2]) / 2.0 / c.dy;
         };
+
+        // detector; the edge record for sleeping-neighbour reads only when the
+        // tile did not fire: a tile that fired is awake next step (the dilation
+        // keeps the tile itself) and a record is read only while its owner
+        // sleeps, so the record a sleeping owner presents is its last step's
+        const int any = dba ? block_any<NT>(flag, sflag) : 0;
+        const bool store = dba && !any;
+        if (store) {
+            gdouble* e = I.aux[0] + (int64_t)t * E::R;
+            // ring primitives
+            for (int k = tid; k < E::RING; k += NT) {
+                int lx, ly;
+                E::cell(k, &lx, &ly);
+                const int q = (lx + 2) + (ly + 2) * W2;
+                e[k] = sP[q];
+                e[E::RING + k] = sP[A2 + q];
+                e[2 * E::RING + k] = sP[2 * A2 + q];
+                e[3 * E::RING + k] = sP[3 * A2 + q];
+            }
+            // left column: primes + x-gradients; bottom row: primes + y-gradients
+            for (int k = tid; k < 2 * 8 * TB; k += NT) {
+                const int part = k / (8 * TB), j = k % (8 * TB), q = j / TB, i = j % TB;
+                const int lx = part == 0 ? 1 : 1 + i, ly = part == 0 ? 1 + i : 1;  // W1 coordinates
+                const double v = q < 4 ? sPR[q * A1 + lx + ly * W1]
+                                       : (part == 0 ? grad_x(q - 4, lx, ly) : grad_y(q - 4, lx, ly));
+                e[(part == 0 ? E::LEFT : E::BOTTOM) + j] = v;
+            }
+        }
 
         // 3a. x-face fluxes of W1 columns 0..TB on the tile rows
         for (int k = tid; k < NF; k += NT) {
@@ -310,7 +320,7 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
                     R[q] = sPR[q * A1 + a] + grad_x(q, lx, ly) * c.dx / 2.0;  // face[1] of the left cell
                 }
                 kh_rusanov(c, L[0], R[0], L[3], R[3], L[1], R[1], L[2], R[2], &fm, &fe, &fu, &fv);
-                if (dba && lx == TB) {
+                if (store && lx == TB) {
                     gdouble* e = I.aux[0] + (int64_t)t * E::R + E::XF + (ly - 1);
                     e[0] = fm;
                     e[TB] = fu;
@@ -351,7 +361,7 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
                 }
                 // momentum roles swapped for y faces (example_Kelvin_Helmholtz/KH_DBA.cu:738-740)
                 kh_rusanov(c, L[0], R[0], L[3], R[3], L[2], R[2], L[1], R[1], &fm, &fe, &fv, &fu);
-                if (dba && ly == TB) {
+                if (store && ly == TB) {
                     gdouble* e = I.aux[0] + (int64_t)t * E::R + E::YF + (lx - 1);
                     e[0] = fm;
                     e[TB] = fu;
@@ -398,11 +408,8 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
             }
         }
 
-        // 5. detector, per-tile speeds, conserved bookkeeping
-        if (dba) {
-            const int any = block_any<NT>(flag, sflag);
-            if (tid == 0) I.nflag[0][t] = (uint8_t)any;
-        }
+        // 5. detector byte, per-tile speeds, conserved bookkeeping
+        if (dba && tid == 0) I.nflag[0][t] = (uint8_t)any;
         if (cfl) {
             const double vs = block_max<NT>(vstale, sred);
             const double vf = block_max<NT>(vfresh, sred);

@@ -69,7 +69,7 @@ class Request:
         return self.side * self.side
 
 
-def make_requests(n, master_seed=7, side_lo=256, side_hi=4096, steps_lo=1000, steps_hi=50000, multiple=32):
+def make_requests(n, master_seed=7, side_lo=256, side_hi=4096, steps_lo=1000, steps_hi=50000, multiple=1):
     """n requests (arrival times unset): 50% dendrite, 50% CH, sides and step
     counts log-uniform. Draws come from splitmix64 streams of master_seed."""
     out = []

@@ -60,3 +60,17 @@ def test_struct_layout_matches_c(tmp_path):
 def test_missing_library_fails_loudly(tmp_path):
     with pytest.raises(OSError):
         abi.load(str(tmp_path / "nope.so"))
+
+
+def test_partial_tile_one_cell_wide_is_refused():
+    """A grid that leaves a partial last tile one cell wide is refused at creation
+    (validation happens before any device work)."""
+    from dba_amd import configs
+    lib = abi.load()
+    p = configs.ch_params(n=65, ny=64, tile=32)
+    # no context is needed to see the refusal: dba_footprint validates too
+    b = C.c_int64()
+    assert lib.dba_footprint(C.byref(p), C.byref(b)) != abi.OK
+    assert "one cell wide" in lib.dba_last_error().decode()
+    p = configs.ch_params(n=66, ny=64, tile=32)
+    assert lib.dba_footprint(C.byref(p), C.byref(b)) == abi.OK

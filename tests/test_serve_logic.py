@@ -74,7 +74,9 @@ def test_request_draws():
     kinds = [r.kind for r in rs]
     assert 0.45 < kinds.count("dendrite") / len(rs) < 0.55
     sides = np.array([r.side for r in rs])
-    assert sides.min() >= 256 and sides.max() <= 4096 and np.all(sides % 32 == 0)
+    assert sides.min() >= 256 and sides.max() <= 4096 and np.all(sides % 8 != 1)
+    # not rounded to the tile: every residue the library accepts occurs
+    assert len(set((sides % 32).tolist())) >= 28
     steps = np.array([r.steps for r in rs])
     assert steps.min() >= 1000 and steps.max() <= 50000
     # log-uniform: the median of log(steps) sits at the middle of [log 1e3, log 5e4]
@@ -95,7 +97,7 @@ def test_sweep_draws():
     items = batch.make_sweep()
     assert len(items) == 64
     for it in items:
-        assert 256 <= it.side <= 2048 and it.side % 32 == 0
+        assert 256 <= it.side <= 2048 and it.side % 8 != 1
         assert 0.01 <= it.delta <= 0.05 and it.aniso_j in (4, 6) and 1.2 <= it.latent_k <= 2.0
         assert 1 <= len(it.seeds) <= 3
         assert all(16 <= c < it.side - 16 for s in it.seeds for c in s)
@@ -110,6 +112,10 @@ def test_log_uniform_side_rounding():
     assert batch.log_uniform_side(0.0, 256, 2048, 32) == 256
     assert batch.log_uniform_side(1.0, 256, 2048, 32) == 2048
     assert batch.log_uniform_side(0.5, 256, 2048, 32) == 736  # sqrt(256*2048) = 724.1 -> 736
+    assert batch.log_uniform_side(0.5, 256, 2048) == 724
+    # 8 k + 1 cells would leave a partial tile one cell wide: one cell more
+    u = math.log(265 / 256) / math.log(2048 / 256)
+    assert batch.log_uniform_side(u, 256, 2048) == 266
 
 
 def test_percentile_nearest_rank():
