@@ -109,6 +109,21 @@ __device__ __forceinline__ int block_any(int pred, int* lds_scratch) {
     return r;
 }
 
+// Workgroup-wide OR of two predicates in one exchange: bit 0 = any(a), bit 1 = any(b).
+template <int NT>
+__device__ __forceinline__ int block_any2(int a, int b, int* lds_scratch) {
+    constexpr int NW = NT / 64;
+    const unsigned long long ba = __ballot(a), bb = __ballot(b);
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) lds_scratch[wid] = (ba != 0ull ? 1 : 0) | (bb != 0ull ? 2 : 0);
+    __syncthreads();
+    int r = 0;
+#pragma unroll
+    for (int w = 0; w < NW; w++) r |= lds_scratch[w];
+    __syncthreads();
+    return r;
+}
+
 // Lane i gets v of lane i + N in its row of 16 lanes (a DPP row shift, no LDS
 // traffic); lanes whose source leaves the row get 0.
 template <int N>

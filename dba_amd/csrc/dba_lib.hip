@@ -101,6 +101,7 @@ struct Group {
     int model = 0, tile = 0, tile_z = 1, threads = 0;
     int dense_threads = 0;        // workgroup size of the single-launch step kernel
     bool conserved = false;       // one conserved instance (single-launch steps carry its mean shift)
+    bool partial = false;         // an instance with partial tiles: the kernels' PART instances
     Inst* d_insts = nullptr;
     Seg* d_segs = nullptr;
     InstSegs* d_isegs = nullptr;
@@ -456,6 +457,8 @@ dba_status get_group(dba_ctx* ctx, const std::vector<int>& ids, Group** out) {
     g->threads = first->p.threads ? first->p.threads : default_threads(g->model, g->tile);
     g->dense_threads = first->p.threads ? first->p.threads : default_dense_threads(g->model, g->tile, g->total_tiles);
     g->conserved = ids.size() == 1 && first->p.mode == DBA_MODE_CONSERVED;
+    for (int id : ids)
+        if (!ctx->inst[id]->h.full_tiles) g->partial = true;
     // Model B: 65536 workgroups (4 blocks each on a fully awake 256^3 field)
     // measured faster than 16384 or 3072 with longer pipelines per wave
     int64_t cap = g->model == DBA_MODEL_MODELB ? 65536 : (g->model == DBA_MODEL_DENDRITE_K94 ? 16384 : 4096);
@@ -507,11 +510,17 @@ void launch_step_tb(const Group* g, hipStream_t s) {
         case DBA_MODEL_CH: {
             // the strip layout needs NT <= TB*TB
             constexpr int NTC = NT <= TB * TB ? NT : TB * TB;
-            hipLaunchKernelGGL((ch_step_kernel<TB, NTC>), grid, dim3(NTC), 0, s, wl_of(g));
+            if (g->partial)
+                hipLaunchKernelGGL((ch_step_kernel<TB, NTC, true>), grid, dim3(NTC), 0, s, wl_of(g));
+            else
+                hipLaunchKernelGGL((ch_step_kernel<TB, NTC, false>), grid, dim3(NTC), 0, s, wl_of(g));
             break;
         }
         case DBA_MODEL_DENDRITE_K93:
-            hipLaunchKernelGGL((k93_step_kernel<TB, NT>), grid, block, 0, s, wl_of(g));
+            if (g->partial)
+                hipLaunchKernelGGL((k93_step_kernel<TB, NT, true>), grid, block, 0, s, wl_of(g));
+            else
+                hipLaunchKernelGGL((k93_step_kernel<TB, NT, false>), grid, block, 0, s, wl_of(g));
             break;
         case DBA_MODEL_KH:
             if constexpr (TB <= 16 && NT <= 256) {
@@ -589,14 +598,24 @@ void launch_dense_tb(const Group* g, hipStream_t s, int first, int step) {
     const dim3 grid((unsigned)g->total_tiles);
     if (g->model == DBA_MODEL_CH) {
         constexpr int NTC = NT <= TB * TB ? NT : TB * TB;
-        if (g->conserved)
-            hipLaunchKernelGGL((ch_dense_kernel<TB, NTC, true>), grid, dim3(NTC), 0, s, g->d_insts, g->d_dmap, first,
-                               step);
+        if (g->conserved && g->partial)
+            hipLaunchKernelGGL((ch_dense_kernel<TB, NTC, true, true>), grid, dim3(NTC), 0, s, g->d_insts, g->d_dmap,
+                               first, step);
+        else if (g->conserved)
+            hipLaunchKernelGGL((ch_dense_kernel<TB, NTC, true, false>), grid, dim3(NTC), 0, s, g->d_insts, g->d_dmap,
+                               first, step);
+        else if (g->partial)
+            hipLaunchKernelGGL((ch_dense_kernel<TB, NTC, false, true>), grid, dim3(NTC), 0, s, g->d_insts, g->d_dmap,
+                               first, step);
         else
-            hipLaunchKernelGGL((ch_dense_kernel<TB, NTC, false>), grid, dim3(NTC), 0, s, g->d_insts, g->d_dmap, first,
-                               step);
+            hipLaunchKernelGGL((ch_dense_kernel<TB, NTC, false, false>), grid, dim3(NTC), 0, s, g->d_insts,
+                               g->d_dmap, first, step);
+    } else if (g->partial) {
+        hipLaunchKernelGGL((k93_dense_kernel<TB, NT, true>), grid, dim3(NT), 0, s, g->d_insts, g->d_dmap, first,
+                           step);
     } else {
-        hipLaunchKernelGGL((k93_dense_kernel<TB, NT>), grid, dim3(NT), 0, s, g->d_insts, g->d_dmap, first, step);
+        hipLaunchKernelGGL((k93_dense_kernel<TB, NT, false>), grid, dim3(NT), 0, s, g->d_insts, g->d_dmap, first,
+                           step);
     }
 }
 

@@ -182,7 +182,9 @@ __device__ __noinline__ int ch_tile_partial(CInst& I, int t, uint64_t rec, NbTab
 // the detector byte here; otherwise the caller has filled the table (then a
 // barrier) and stores the returned detector bit any(|lap c| > eps) (0 in regular
 // mode, uniform over the workgroup) itself.
-template <int TB, int NT, bool NINE, bool WORKLIST>
+// PART: the kernel serves instances with partial tiles (a separate instance of
+// the kernels, so the partial path's registers do not cost full-tile grids).
+template <int TB, int NT, bool NINE, bool WORKLIST, bool PART = false>
 __device__ __forceinline__ int ch_tile(CInst& I, int t, uint64_t rec, NbTable& nb, double* sc, double* smu,
                                        double* sred, int* sflag) {
     constexpr int W2 = TB + 4;
@@ -194,7 +196,7 @@ __device__ __forceinline__ int ch_tile(CInst& I, int t, uint64_t rec, NbTable& n
     const int tx = tid % TB, ty = tid / TB;
     const int nx = I.nx, ny = I.ny, ntx = I.ntx, bcx = I.bc_x, bcy = I.bc_y;
     const int x0 = (t % ntx) * TB, y0 = (t / ntx) * TB;
-    if (!I.full_tiles && (x0 + TB > nx || y0 + TB > ny))
+    if (PART && !I.full_tiles && (x0 + TB > nx || y0 + TB > ny))
         return ch_tile_partial<TB, NT, NINE, WORKLIST>(I, t, rec, nb, sc, smu, sred, sflag);
     const bool regular = I.mode == 0;
     const bool cons = I.mode == 2;
@@ -308,7 +310,7 @@ __device__ __forceinline__ int ch_tile(CInst& I, int t, uint64_t rec, NbTable& n
     return any;
 }
 
-template <int TB, int NT>
+template <int TB, int NT, bool PART>
 __global__ __launch_bounds__(NT) void ch_step_kernel(const WorkList wl) {
     static_assert(NT % TB == 0 && TB * TB % NT == 0, "thread layout needs TB | NT | TB*TB");
     constexpr int W2 = TB + 4;
@@ -328,8 +330,8 @@ __global__ __launch_bounds__(NT) void ch_step_kernel(const WorkList wl) {
         uint64_t rec;
         if (!work_next(wl, cur, total, &Ip, &t, &rec)) break;
         CInst& I = *Ip;
-        if (I.stencil != 5) ch_tile<TB, NT, true, true>(I, t, rec, nb, sc, smu, sred, sflag);
-        else ch_tile<TB, NT, false, true>(I, t, rec, nb, sc, smu, sred, sflag);
+        if (I.stencil != 5) ch_tile<TB, NT, true, true, PART>(I, t, rec, nb, sc, smu, sred, sflag);
+        else ch_tile<TB, NT, false, true, PART>(I, t, rec, nb, sc, smu, sred, sflag);
     }
 }
 
@@ -342,7 +344,7 @@ __global__ __launch_bounds__(NT) void ch_step_kernel(const WorkList wl) {
 // worklist kernel. The canonicalising kernels (dense_canon_*) restore the
 // bitmaps and counters at the end of the run. Results are bit-identical to the
 // worklist path.
-template <int TB, int NT, bool CONS>
+template <int TB, int NT, bool CONS, bool PART>
 __global__ __launch_bounds__(NT) void ch_dense_kernel(const Inst* __restrict__ insts,
                                                       const int32_t* __restrict__ dmap, int first, int s) {
     constexpr int W2 = TB + 4;
@@ -398,15 +400,17 @@ __global__ __launch_bounds__(NT) void ch_dense_kernel(const Inst* __restrict__ i
     const int pself = nb.par[0][4], ran = nb.act[0][4];
     int any = 0;
     if (ran)
-        any = I.stencil != 5 ? ch_tile<TB, NT, true, false>(I, t, 0, nb, sc, smu, sred, sflag)
-                             : ch_tile<TB, NT, false, false>(I, t, 0, nb, sc, smu, sred, sflag);
+        any = I.stencil != 5 ? ch_tile<TB, NT, true, false, PART>(I, t, 0, nb, sc, smu, sred, sflag)
+                             : ch_tile<TB, NT, false, false, PART>(I, t, 0, nb, sc, smu, sred, sflag);
     if (threadIdx.x == 0) {
         I.dF[next][t] = (uint8_t)any;
         I.dP[next][t] = (uint8_t)(pself ^ ran);
         if (ran) {
             I.druns[t] += 1;
-            const int64_t step = I.ctr[0] + s;
-            if (step < I.hist_cap) atomicAdd((int*)&I.hist[2 * step], 1);
+            if (I.hist_cap > 0) {
+                const int64_t step = I.ctr[0] + s;
+                if (step < I.hist_cap) atomicAdd((int*)&I.hist[2 * step], 1);
+            }
         }
         if (cons) {
             I.tpart[(int64_t)(1 + next) * I.nt + t] = ran ? nb.tot : 0.0;

@@ -162,9 +162,8 @@ __device__ __forceinline__ int k93_tile(CInst& I, int t, uint64_t rec, NbTable& 
     }
     int bits = 0;
     if (!regular) {
-        const int anyp = block_any<NT>(fp, sflag);
-        const int anyT = block_any<NT>(fT, sflag);
-        bits = anyp | anyT << 1;
+        bits = block_any2<NT>(fp, fT, sflag);
+        const int anyp = bits & 1, anyT = bits >> 1;
         if (WORKLIST && tid == 0) {
             if (ap) I.nflag[0][t] = (uint8_t)anyp;
             if (at) I.nflag[1][t] = (uint8_t)anyT;
@@ -174,13 +173,15 @@ __device__ __forceinline__ int k93_tile(CInst& I, int t, uint64_t rec, NbTable& 
     return bits;
 }
 
-// A partial last tile of an instance whose grid is not a multiple of the tile.
-template <int TB>
+// A partial last tile of an instance whose grid is not a multiple of the tile
+// (only in the kernels' PART instances, which groups holding such an instance
+// use, so the partial path's registers do not cost full-tile grids).
+template <int TB, bool PART>
 __device__ __forceinline__ bool k93_partial(CInst& I, int t) {
-    return !I.full_tiles && ((t % I.ntx) * TB + TB > I.nx || (t / I.ntx) * TB + TB > I.ny);
+    return PART && !I.full_tiles && ((t % I.ntx) * TB + TB > I.nx || (t / I.ntx) * TB + TB > I.ny);
 }
 
-template <int TB, int NT>
+template <int TB, int NT, bool PART>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k93_step_kernel(const WorkList wl) {
     __shared__ K93Lds<TB> L;
     __shared__ int sflag[NT / 64 + 1];
@@ -194,7 +195,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k9
         int t;
         uint64_t rec;
         if (!work_next(wl, cur, total, &Ip, &t, &rec)) break;
-        if (k93_partial<TB>(*Ip, t)) k93_tile<TB, NT, true, true>(*Ip, t, rec, nb, L, sflag);
+        if (k93_partial<TB, PART>(*Ip, t)) k93_tile<TB, NT, true, true>(*Ip, t, rec, nb, L, sflag);
         else k93_tile<TB, NT, true>(*Ip, t, rec, nb, L, sflag);
     }
 }
@@ -202,7 +203,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k9
 // Single-launch step of a K93 group (see ch_dense_kernel): per
 // set, awake bits and parities of the 3x3 owner tiles from the bitmaps (first
 // step) or the previous step's bytes; the tile runs when either set is awake.
-template <int TB, int NT>
+template <int TB, int NT, bool PART>
 __global__ __launch_bounds__(NT) void k93_dense_kernel(const Inst* __restrict__ insts,
                                                        const int32_t* __restrict__ dmap, int first, int s) {
     __shared__ K93Lds<TB> L;
@@ -218,7 +219,7 @@ __global__ __launch_bounds__(NT) void k93_dense_kernel(const Inst* __restrict__ 
     const int ap = nb.act[0][4], at = nb.act[1][4];
     int bits = 0;
     if (ap | at)
-        bits = k93_partial<TB>(I, t) ? k93_tile<TB, NT, false, true>(I, t, 0, nb, L, sflag)
+        bits = k93_partial<TB, PART>(I, t) ? k93_tile<TB, NT, false, true>(I, t, 0, nb, L, sflag)
                                      : k93_tile<TB, NT, false>(I, t, 0, nb, L, sflag);
     if (threadIdx.x == 0) {
         I.dF[next][t] = (uint8_t)(ap ? bits & 1 : 0);
@@ -227,10 +228,12 @@ __global__ __launch_bounds__(NT) void k93_dense_kernel(const Inst* __restrict__ 
         I.dP[next][nt + t] = (uint8_t)(nb.par[1][4] ^ at);
         if (ap | at) {
             I.druns[t] += 1 + ap;  // field-set updates: T over A_p | A_T, p over A_p
-            const int64_t step = I.ctr[0] + s;
-            if (step < I.hist_cap) {
-                if (ap) atomicAdd((int*)&I.hist[2 * step], 1);
-                if (at) atomicAdd((int*)&I.hist[2 * step + 1], 1);
+            if (I.hist_cap > 0) {
+                const int64_t step = I.ctr[0] + s;
+                if (step < I.hist_cap) {
+                    if (ap) atomicAdd((int*)&I.hist[2 * step], 1);
+                    if (at) atomicAdd((int*)&I.hist[2 * step + 1], 1);
+                }
             }
         }
     }
