@@ -102,6 +102,8 @@ struct Group {
     int dense_threads = 0;        // workgroup size of the single-launch step kernel
     bool conserved = false;       // one conserved instance (single-launch steps carry its mean shift)
     bool partial = false;         // an instance with partial tiles: the kernels' PART instances
+    int march = 0;                // column-march single-launch steps: tiles per strip (0: off)
+    bool nine = true;             // CH 9-point stencil (the march kernel is instanced on it)
     Inst* d_insts = nullptr;
     Seg* d_segs = nullptr;
     InstSegs* d_isegs = nullptr;
@@ -459,6 +461,19 @@ dba_status get_group(dba_ctx* ctx, const std::vector<int>& ids, Group** out) {
     g->conserved = ids.size() == 1 && first->p.mode == DBA_MODE_CONSERVED;
     for (int id : ids)
         if (!ctx->inst[id]->h.full_tiles) g->partial = true;
+    // Column-march single-launch steps (ch_march_kernel): one large CH instance of
+    // full 32 x 32 tiles, periodic both ways, not conserved; K tiles per strip,
+    // as many as keep >= 2048 strips in flight
+    g->nine = first->p.stencil != 5;
+    if (ids.size() == 1 && g->model == DBA_MODEL_CH && g->tile == 32 && g->dense_threads == 256 &&
+        first->p.mode != DBA_MODE_CONSERVED && first->h.full_tiles && first->p.bc_x == DBA_BC_PERIODIC &&
+        first->p.bc_y == DBA_BC_PERIODIC && first->h.nt >= kMarchMinTiles) {
+        for (int k : {8, 4, 2})
+            if (first->h.nty % k == 0 && first->h.nt / k >= 2048) {
+                g->march = k;
+                break;
+            }
+    }
     // Model B: 65536 workgroups (4 blocks each on a fully awake 256^3 field)
     // measured faster than 16384 or 3072 with longer pipelines per wave
     int64_t cap = g->model == DBA_MODEL_MODELB ? 65536 : (g->model == DBA_MODEL_DENDRITE_K94 ? 16384 : 4096);
@@ -593,6 +608,15 @@ bool single_launch_ok(const dba_ctx* ctx, const Group* g) {
     return true;
 }
 
+template <int K>
+void launch_march(const Group* g, hipStream_t s, int first, int step) {
+    const dim3 grid((unsigned)(g->total_tiles / K));
+    if (g->nine)
+        hipLaunchKernelGGL((ch_march_kernel<K, true>), grid, dim3(256), 0, s, g->d_insts, first, step);
+    else
+        hipLaunchKernelGGL((ch_march_kernel<K, false>), grid, dim3(256), 0, s, g->d_insts, first, step);
+}
+
 template <int TB, int NT>
 void launch_dense_tb(const Group* g, hipStream_t s, int first, int step) {
     const dim3 grid((unsigned)g->total_tiles);
@@ -631,6 +655,14 @@ void launch_dense_nt(const Group* g, hipStream_t s, int first, int step) {
 }
 
 void launch_dense(const Group* g, hipStream_t s, int first, int step) {
+    if (g->march) {
+        switch (g->march) {
+            case 8: launch_march<8>(g, s, first, step); break;
+            case 4: launch_march<4>(g, s, first, step); break;
+            default: launch_march<2>(g, s, first, step); break;
+        }
+        return;
+    }
     switch (g->tile) {
         case 8: launch_dense_nt<8>(g, s, first, step); break;
         case 16: launch_dense_nt<16>(g, s, first, step); break;

@@ -420,3 +420,262 @@ __global__ __launch_bounds__(NT) void ch_dense_kernel(const Inst* __restrict__ i
 }
 
 }  // namespace dba
+
+namespace dba {
+
+// ---- Column-march single-launch step for large CH grids ----
+//
+// One workgroup walks a strip of K tiles up one tile column (32 x 32 tiles, 256
+// threads, periodic both ways, one instance, not conserved). The c rows of the
+// strip live in a 64-row LDS ring: tile j reads ring rows 32 j .. 32 j + 35
+// (its 32 rows and 2 halo rows below and above), and the next tile reuses the
+// 4 rows the two share, so every row of the strip is read from memory once.
+// While tile j computes, the 32 new rows of tile j + 1 (with their 2-cell x
+// halo) are already loading into registers; the barriers between the phases
+// order LDS only, so those loads stay in flight across them. Per tile the
+// arithmetic, the operation order, the detector, the stale mu of sleeping
+// neighbours and the bytes written are those of ch_dense_kernel, so the two
+// give identical results.
+
+// Workgroup barrier that orders LDS only: outstanding global loads stay in flight.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <int NT>
+__device__ __forceinline__ int block_any_lds(int pred, int* lds_scratch) {
+    constexpr int NW = NT / 64;
+    const unsigned long long b = __ballot(pred);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) lds_scratch[wid] = b != 0ull;
+    lds_barrier();
+    int r = 0;
+#pragma unroll
+    for (int w = 0; w < NW; w++) r |= lds_scratch[w];
+    lds_barrier();
+    return r;
+}
+
+constexpr int kMarchRing = 64;  // LDS ring rows (a power of two >= 36 + 32 - 4)
+constexpr int kMarchMinTiles = 8192;  // grids from 2896^2 (32 x 32 tiles) march
+
+// 9- or 5-point Laplacian at window cell (x, y) of the ring (window row 0 at ring row base)
+template <bool NINE>
+__device__ __forceinline__ double ring_lap(const double* ring, int base, int x, int y, double inv) {
+    constexpr int W2 = 36;
+    const double* rm = ring + ((base + y - 1) & (kMarchRing - 1)) * W2;
+    const double* r0 = ring + ((base + y) & (kMarchRing - 1)) * W2;
+    const double* rp = ring + ((base + y + 1) & (kMarchRing - 1)) * W2;
+    const double s1 = (r0[x + 1] + r0[x - 1]) + (rp[x] + rm[x]);
+    if (NINE) {
+        const double s2 = (rp[x + 1] + rp[x - 1]) + (rm[x + 1] + rm[x - 1]);
+        return ((4.0 * s1 + s2) - 20.0 * r0[x]) * inv;
+    }
+    return (s1 - 4.0 * r0[x]) * inv;
+}
+
+template <int K, bool NINE>
+__global__ __launch_bounds__(256) void ch_march_kernel(const Inst* __restrict__ insts, int first, int s) {
+    constexpr int TB = 32, NT = 256, R = 4;
+    constexpr int W2 = TB + 4, W1 = TB + 2;
+    constexpr int NH1 = W1 * W1 - TB * TB;
+    constexpr int NC = 3 * (K + 2);  // columns (left, own, right) x tile rows ty0 - 1 .. ty0 + K
+    constexpr int MASK = kMarchRing - 1;
+    __shared__ double ring[kMarchRing * W2];
+    __shared__ double smu[W1 * W1];
+    __shared__ int s_tile[NC], s_par[NC], s_act[NC];
+    __shared__ int s_runs[K];
+    __shared__ int sflag[NT / 64 + 1];
+    CInst& I = ((CInst*)insts)[0];
+    const int tid = threadIdx.x, tx = tid % TB, ty = tid / TB;
+    const int nx = I.nx, ny = I.ny, ntx = I.ntx, nty = I.nty;
+    const int prev = s & 1, next = prev ^ 1;
+    const bool regular = I.mode == 0;
+    const gdouble* __restrict__ b0 = I.buf[0][0];
+    const gdouble* __restrict__ b1 = I.buf[1][0];
+    gdouble* __restrict__ mu_store = I.aux[0];
+    const double kappa = I.kappa, eps = I.thresh_a;
+    const double inv = NINE ? I.inv6dx2 : I.invdx2;
+    const double dt = I.dt, mob = I.mobility, rg = I.react_gamma, cbar = I.cbar;
+    const int item = xcd_deal(blockIdx.x, gridDim.x, 0, gridDim.x);
+    const int sx = item % ntx, ty0 = (item / ntx) * K;
+    const int x0 = sx * TB;
+
+    // 0. owner, buffer parity and awake bit of the tiles around the strip
+    if (tid < NC) {
+        const int c = tid % 3, r = tid / 3;
+        const int oi = sx + c - 1 < 0 ? ntx - 1 : (sx + c - 1 >= ntx ? 0 : sx + c - 1);
+        const int oj0 = ty0 + r - 1;
+        const int oj = oj0 < 0 ? oj0 + nty : (oj0 >= nty ? oj0 - nty : oj0);
+        const int o = oi + oj * ntx;
+        int par, act;
+        if (first) {
+            par = bit_of(I.par[0], o);
+            act = regular ? 1 : bit_of(I.a_cur[0], o);
+        } else {
+            par = I.dP[prev][o];
+            act = regular ? 1 : dilated2d(I.dF[prev], oi, oj, ntx, nb2_of(oi, oj, ntx, nty, 0, 0), 0);
+        }
+        s_tile[tid] = o;
+        s_par[tid] = par;
+        s_act[tid] = act;
+    } else if (tid < NC + K) {
+        // the strip's per-tile update counts, read now so no tile waits on them
+        s_runs[tid - NC] = I.druns[sx + (ty0 + tid - NC) * ntx];
+    }
+    __syncthreads();
+
+    // Rows [L, L + 32) of the strip's window (global row ty0 * 32 - 2 + L): the
+    // thread's 4 rows of its column, and (threads 0..127) one x-halo cell.
+    const int hq = tid >> 2, hc = tid & 3, hlx = hc < 2 ? hc : TB + hc;  // halo row, LDS column
+    const int hmx = map_index(x0 + hlx - 2, nx, 0);
+    auto row_of = [&](int L) { return map_index(ty0 * TB - 2 + L, ny, 0); };
+    auto trow = [&](int L) { return (L + TB - 2) / TB; };  // table row of window row L
+    double pm[R], ph = 0.0;
+    auto fetch = [&](int L) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int Lr = L + ty * R + r;
+            const gdouble* src = s_par[1 + 3 * trow(Lr)] ? b1 : b0;
+            pm[r] = src[x0 + tx + (int64_t)row_of(Lr) * nx];
+        }
+        if (tid < 4 * TB) {
+            const int Lr = L + hq;
+            const gdouble* src = s_par[(hc < 2 ? 0 : 2) + 3 * trow(Lr)] ? b1 : b0;
+            ph = src[hmx + (int64_t)row_of(Lr) * nx];
+        }
+    };
+    auto deposit = [&](int L) {
+#pragma unroll
+        for (int r = 0; r < R; r++) ring[((L + ty * R + r) & MASK) * W2 + tx + 2] = pm[r];
+        if (tid < 4 * TB) ring[((L + hq) & MASK) * W2 + hlx] = ph;
+    };
+    // the 4 window rows below the first block (2 halo rows, the first tile's rows 0-1)
+    if (tid < 4 * W2) {
+        const int L = tid / W2, lx = tid % W2;
+        const int c = lx < 2 ? 0 : (lx < TB + 2 ? 1 : 2);
+        const gdouble* src = s_par[c + 3 * trow(L)] ? b1 : b0;
+        ring[(L & MASK) * W2 + lx] = src[map_index(x0 + lx - 2, nx, 0) + (int64_t)row_of(L) * nx];
+    }
+    // block j covers window rows [32 j + 4, 32 j + 36): tile j's rows 2-31 and
+    // tile j + 1's rows 0-1; it is needed when tile j or tile j + 1 runs
+    auto needed = [&](int j) { return (s_act[1 + 3 * (j + 1)] | s_act[1 + 3 * (j + 2)]) != 0; };
+    if (needed(0)) {
+        fetch(4);
+        deposit(4);
+    }
+    __syncthreads();
+
+#pragma unroll 1
+    for (int j = 0; j < K; j++) {
+        const bool pre = j + 1 < K && needed(j + 1);
+        if (pre) fetch(32 * (j + 1) + 4);  // in flight while tile j computes
+        const int tr = j + 1;               // table row of tile j
+        const int t = sx + (ty0 + j) * ntx;
+        const int pself = s_par[1 + 3 * tr];
+        const int ran = s_act[1 + 3 * tr];
+        const int base = 32 * j;            // ring row of window row 0 of tile j
+        int any = 0;
+        if (ran) {
+            gdouble* __restrict__ nxt = pself ? I.buf[0][0] : I.buf[1][0];
+            // mu on the strip (+ detector), then on the 1-cell ring
+            double c[R];
+            int flag = 0;
+            {
+                double Lc[R + 2], Cc[R + 2], Rc[R + 2];
+#pragma unroll
+                for (int r = 0; r < R + 2; r++) {
+                    const double* row = ring + ((base + ty * R + 1 + r) & MASK) * W2;
+                    Lc[r] = row[tx + 1];
+                    Cc[r] = row[tx + 2];
+                    Rc[r] = row[tx + 3];
+                }
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const double s1 = (Rc[r + 1] + Lc[r + 1]) + (Cc[r + 2] + Cc[r]);
+                    double lc;
+                    if (NINE) {
+                        const double s2 = (Rc[r + 2] + Lc[r + 2]) + (Rc[r] + Lc[r]);
+                        lc = ((4.0 * s1 + s2) - 20.0 * Cc[r + 1]) * inv;
+                    } else {
+                        lc = (s1 - 4.0 * Cc[r + 1]) * inv;
+                    }
+                    const double cv = Cc[r + 1];
+                    c[r] = cv;
+                    smu[(tx + 1) + (ty * R + r + 1) * W1] = (cv * cv * cv - cv) - kappa * lc;
+                    flag |= fabs(lc) > eps;
+                }
+            }
+            for (int k = tid; k < NH1; k += NT) {
+                int lx, ly;
+                if (k < W1) { ly = 0; lx = k; }
+                else if (k < 2 * W1) { ly = TB + 1; lx = k - W1; }
+                else { const int q = k - 2 * W1; ly = 1 + q / 2; lx = (q % 2) ? TB + 1 : 0; }
+                const int dxi = nb_dir1<TB, 1>(lx), dyi = nb_dir1<TB, 1>(ly);
+                const int e = dxi + 3 * (tr + dyi - 1);
+                double mu;
+                if (s_act[e]) {
+                    const double lcv = ring_lap<NINE>(ring, base, lx + 1, ly + 1, inv);
+                    const double cv = ring[((base + ly + 1) & MASK) * W2 + lx + 1];
+                    mu = (cv * cv * cv - cv) - kappa * lcv;
+                } else {
+                    const int mx = map_index(x0 + lx - 1, nx, 0);
+                    const int my = map_index((ty0 + j) * TB + ly - 1, ny, 0);
+                    mu = TileRing<TB>::stale(mu_store, s_tile[e], mx, my, nx, ny);
+                }
+                smu[lx + ly * W1] = mu;
+            }
+            lds_barrier();
+            // c' on the strip
+            {
+                double Lm[R + 2], Cm[R + 2], Rm[R + 2];
+#pragma unroll
+                for (int r = 0; r < R + 2; r++) {
+                    const int row = (ty * R + r) * W1;
+                    Lm[r] = smu[row + tx];
+                    Cm[r] = smu[row + tx + 1];
+                    Rm[r] = smu[row + tx + 2];
+                }
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const double s1 = (Rm[r + 1] + Lm[r + 1]) + (Cm[r + 2] + Cm[r]);
+                    double lm;
+                    if (NINE) {
+                        const double s2 = (Rm[r + 2] + Lm[r + 2]) + (Rm[r] + Lm[r]);
+                        lm = ((4.0 * s1 + s2) - 20.0 * Cm[r + 1]) * inv;
+                    } else {
+                        lm = (s1 - 4.0 * Cm[r + 1]) * inv;
+                    }
+                    const double change = dt * (mob * lm - rg * (c[r] - cbar));
+                    nxt[x0 + tx + (int64_t)((ty0 + j) * TB + ty * R + r) * nx] = c[r] + change;
+                }
+            }
+            any = regular ? 0 : block_any_lds<NT>(flag, sflag);
+            if (!regular && !any) {
+                for (int k = tid; k < TileRing<TB>::N; k += NT) {
+                    int lx, ly;
+                    TileRing<TB>::cell(k, &lx, &ly);
+                    mu_store[(int64_t)t * TileRing<TB>::N + k] = smu[(lx + 1) + (ly + 1) * W1];
+                }
+            }
+        }
+        if (tid == 0) {
+            I.dF[next][t] = (uint8_t)any;
+            I.dP[next][t] = (uint8_t)(pself ^ ran);
+            if (ran) {
+                I.druns[t] = s_runs[j] + 1;
+                if (I.hist_cap > 0) {
+                    const int64_t step = I.ctr[0] + s;
+                    if (step < I.hist_cap) atomicAdd((int*)&I.hist[2 * step], 1);
+                }
+            }
+        }
+        lds_barrier();  // every read of tile j's window and mu is done
+        if (pre) deposit(32 * (j + 1) + 4);
+        lds_barrier();
+    }
+}
+
+}  // namespace dba
