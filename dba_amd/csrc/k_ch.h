@@ -459,7 +459,7 @@ __device__ __forceinline__ int block_any_lds(int pred, int* lds_scratch) {
 }
 
 constexpr int kMarchRing = 64;  // LDS ring rows (a power of two >= 36 + 32 - 4)
-constexpr int kMarchMinTiles = 8192;  // grids from 2896^2 (32 x 32 tiles) march
+constexpr int kMarchMinTiles = 4096;  // grids from 2048^2 (32 x 32 tiles) march
 
 // 9- or 5-point Laplacian at window cell (x, y) of the ring (window row 0 at ring row base)
 template <bool NINE>

@@ -440,14 +440,16 @@ __global__ __launch_bounds__(kFinNT) void finalize_merge_kernel(const Inst* __re
     finalize_merge_body<kFinNT>(insts, nseg, isegs, n_inst, parts, counts, offsets, rotate, si);
 }
 
-// Instance and tile of a single-launch step's workgroup. Workgroups are dealt
-// to XCDs in contiguous ranges (the worklist kernels' deal), so halo rows a
-// neighbour tile wrote last step are in this XCD's L2. A group of several
+// Instance and tile of a single-launch step's workgroup. With `deal`, workgroups
+// are dealt to XCDs in contiguous ranges (the worklist kernels' deal), so halo
+// rows a neighbour tile wrote last step are in this XCD's L2 (CH, every tile
+// active); without, tile b goes to XCD b % 8, which spreads a sparse set of
+// awake tiles over all XCDs (K93, whose awake tiles cluster in a few rows). A group of several
 // instances or segments maps the dealt index through dmap (instance << 24 |
 // tile, the instances' tiles in group order); one instance in one segment
 // uses the index itself.
-__device__ __forceinline__ CInst& dense_item(const Inst* insts, const int32_t* dmap, int* t) {
-    const int b = xcd_deal(blockIdx.x, gridDim.x, 0, gridDim.x);
+__device__ __forceinline__ CInst& dense_item(const Inst* insts, const int32_t* dmap, int* t, bool deal = true) {
+    const int b = deal ? xcd_deal(blockIdx.x, gridDim.x, 0, gridDim.x) : (int)blockIdx.x;
     if (!dmap) {
         *t = b;
         return ((CInst*)insts)[0];
