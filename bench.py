@@ -415,15 +415,18 @@ def bench_serve(args):
         srv = serve.Server(be, budget, chunk=args.chunk, tiles=tiles, target_occupancy=0.8, gain=2.0, log=log,
                            modes=modes)
         done = srv.run(rs, rate=args.rate)
-        rep = serve.report(done, srv.wall, srv.occupancy)
+        rep = serve.report(done, srv.wall, srv.occupancy, busy_s=srv.busy_s)
         log(f"served {rep['completed']} in {srv.wall:.1f} s, p50 {rep['p50_latency_s']:.3f} s, "
-            f"p99 {rep['p99_latency_s']:.3f} s, {rep['active_rate']:.4g} active/s")
-        # sequential loops: the first requests of the stream alone, until the time budget
+            f"p99 {rep['p99_latency_s']:.3f} s, {rep['active_rate']:.4g} active/s, "
+            f"device busy {rep['device_busy_fraction']:.3f}")
+        # sequential loops: requests of the whole stream alone, in van der Corput
+        # order (any prefix spans the stream), until the time budget
         seq = {}
+        order = serve.spread_order(len(rs))
         for mode in ("graph", "plain"):
             seq_t = seq_u = 0.0
             nseq = 0
-            for r in rs:
+            for r in (rs[i] for i in order):
                 if seq_t > args.seq_budget_s:
                     break
                 inst = ctx.create(r.params(tiles, {}))
@@ -460,10 +463,12 @@ def bench_serve(args):
         "served_vs_sequential_plain": rep["active_rate"] / seq["plain"][0],
         "served_vs_sequential_graph": rep["active_rate"] / seq["graph"][0],
         "sequential_plain": {"value": seq["plain"][0], "requests": seq["plain"][1], "wall_s": seq["plain"][2],
-                             "note": "the first requests of the same stream run alone, one step launched and "
-                                     "waited for per time step (the reference loop's shape)"},
+                             "note": "requests of the same stream run alone, in van der Corput order over the "
+                                     "whole stream until the time budget, one step launched and waited for per "
+                                     "time step (the reference loop's shape)"},
         "sequential_graph": {"value": seq["graph"][0], "requests": seq["graph"][1], "wall_s": seq["graph"][2],
-                             "note": "the same requests alone, graph-replayed chunks of steps"},
+                             "note": "the same sample alone, graph-replayed chunks of steps"},
+        "host_wait_s": srv.sync_s,
         "ticks": srv.ticks, "max_used_bytes": srv.max_used,
         "final_rate_per_s": srv.rate_trace[-1][1] if srv.rate_trace else args.rate,
     }

@@ -195,8 +195,8 @@ def autotune(ctx, items, trial_steps=200, warm_steps=20, tiles=TILES, threads=TH
     table = []
     try:
         for tile in tiles:
-            if any(it.side % tile for it in items):
-                continue
+            if any(it.side % tile == 1 or it.side < tile for it in items):
+                continue  # the library refuses a partial tile one cell wide
             for nt in threads:
                 for mode in modes:
                     ctx.set_single_launch(mode)

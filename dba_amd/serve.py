@@ -96,6 +96,19 @@ def poisson_times(n, rate, seed=11):
     return np.cumsum(-np.log1p(-u) / rate)
 
 
+def spread_order(n):
+    """0 .. n-1 in van der Corput (bit-reversed) order: every prefix is spread over
+    the whole range, so a time-limited pass over requests in this order samples the
+    whole stream rather than its first requests."""
+    bits = max(1, (n - 1).bit_length())
+    out = []
+    for k in range(1 << bits):
+        r = int(format(k, "0%db" % bits)[::-1], 2)
+        if r < n:
+            out.append(r)
+    return out
+
+
 def percentile(values, q):
     """Nearest-rank percentile (q in [0, 100]) of a non-empty sequence."""
     v = sorted(values)
@@ -148,6 +161,18 @@ class LibraryBackend:
     def sync(self):
         self.ctx.sync()
 
+    # GPU time of a tick: events on the library stream around the tick's launches
+    # (read after the tick's sync; an idle GPU runs the first event at once, so
+    # the span is the tick's busy time)
+    def tick_begin(self):
+        self.ctx.marker(4)
+
+    def tick_end(self):
+        self.ctx.marker(5)
+
+    def tick_gpu_s(self):
+        return self.ctx.elapsed_ms(4, 5) / 1e3
+
     def updates(self, h):
         return h.info().cell_updates
 
@@ -177,6 +202,8 @@ class Server:
         self.log = log
         self.used = 0
         self.max_used = 0
+        self.busy_s = 0.0     # GPU time of the ticks (backends that time them)
+        self.sync_s = 0.0     # host time spent waiting for ticks
         self.occupancy = []   # (tick wall time, used / capacity)
         self.ticks = 0
         self.rate_trace = []
@@ -244,19 +271,28 @@ class Server:
                          f"used={self.used / self.capacity:.2f}" + (f" rate={rate:.2f}/s" if rate else ""))
             arrivals(now)
             k = 0
+            timed = hasattr(self.b, "tick_begin")
             if resident:
                 # one tick for the resident set, asynchronous on the device
                 k = min(self.chunk, min(v[1] for v in resident.values()))
+                if timed:
+                    self.b.tick_begin()
                 if self.modes is None:
                     self.b.step([v[0] for v in resident.values()], k, self.chunk)
                 else:
                     for kind in sorted({v[2].kind for v in resident.values()}):
                         hs = [v[0] for v in resident.values() if v[2].kind == kind]
                         self.b.step(hs, k, self.chunk, self.modes.get(kind, 1))
+                if timed:
+                    self.b.tick_end()
             # admissions overlap the tick: new instances are set up on another stream
             incoming = admit()
             if k:
+                ts = clock()
                 self.b.sync()
+                self.sync_s += clock() - ts
+                if timed:
+                    self.busy_s += self.b.tick_gpu_s()
                 self.ticks += 1
                 now = clock() - t0
                 self.occupancy.append((now, self.used / self.capacity))
@@ -284,7 +320,7 @@ class Server:
         return done
 
 
-def report(done, wall, occupancy):
+def report(done, wall, occupancy, busy_s=None):
     lat = [c.latency for c in done]
     upd = sum(c.active_updates for c in done)
     full = sum(c.cells * c.steps for c in done)
@@ -301,4 +337,5 @@ def report(done, wall, occupancy):
         "occupancy_p10": float(np.percentile(occ, 10)),
         "occupancy_p90": float(np.percentile(occ, 90)),
         "wall_s": wall,
+        "device_busy_fraction": (busy_s / wall) if (busy_s is not None and wall > 0) else None,
     }

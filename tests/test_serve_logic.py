@@ -212,3 +212,15 @@ def test_report_fields():
     assert rep["p50_latency_s"] == 5.0 and rep["p99_latency_s"] == 10.0
     assert rep["active_rate"] == 500.0 and rep["full_rate"] == 1000.0
     assert abs(rep["occupancy_mean"] - 0.7) < 1e-12
+
+
+def test_spread_order_covers_and_spreads():
+    for n in (1, 2, 7, 300, 1024):
+        o = serve.spread_order(n)
+        assert sorted(o) == list(range(n))
+    o = serve.spread_order(300)
+    # any prefix of k entries reaches across the stream: its largest gap is about 2 n / k
+    for k in (8, 16, 50):
+        pre = sorted(o[:k])
+        gaps = np.diff([0] + pre + [300])
+        assert gaps.max() <= 2 * 300 // k + 2, (k, gaps.max())
