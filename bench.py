@@ -28,6 +28,10 @@ counter of cells in awake tiles).
 Before the timed region the graph it replays is captured, instantiated and
 replayed once, whatever --warmup is, so the timed region only replays it.
 
+The default workload's line also carries "companions": the same measurement of
+dendrite, ch_cons, kh, modelb and dendrite3d at their own step counts, in the
+same process after the default one (--no-companions skips them).
+
 roofline: the step kernel's algorithmic bytes per launch (B_alg x active cells
 of that launch: CH fused pass 16 B = read c + write c'; dendrite 16 B per (cell,
 field) update; KH 64 B = read and write the 4 conserved fields) over its average
@@ -313,8 +317,9 @@ def copy_calibration(n=4096, reps=8):
 
 def traffic_from_profile(workload):
     """HBM bytes per step-kernel launch measured by the rocprofv3 --pmc passes
-    committed in profiles/ (FETCH_SIZE + WRITE_SIZE, scaled by the copy
-    calibration), or None when no such measurement is committed."""
+    committed in profiles/ (FETCH_SIZE scaled by the same run's finite_kernel
+    read of known size, plus WRITE_SIZE), or None when no such measurement is
+    committed."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.md")
     if not os.path.exists(path):
         return None
@@ -475,6 +480,136 @@ def bench_serve(args):
     return out
 
 
+# The other single-instance configurations, measured after the default one in the
+# same process (their own step counts, no CPU baseline): the dendrite of the
+# plain-DBA configuration (sleeping tiles), conserved CH, KH, and the reference's
+# own 3D models.
+COMPANIONS = ("dendrite", "ch_cons", "kh", "modelb", "dendrite3d")
+
+
+def companion(o):
+    """The fields of a companion measurement kept in the default line."""
+    r = o["roofline"]
+    return {"workload": o["config"]["workload"], "value": o["value"], "unit": o["unit"], "steps": o["steps"],
+            "warmup": o["warmup"], "ms_per_step": o["ms_per_step"], "gpu_ms_per_step": o["gpu_ms_per_step"],
+            "active_fraction": o["active_fraction"], "full_grid_updates_per_s": o["full_grid_updates_per_s"],
+            "finite": o["finite"],
+            "roofline": {k: r[k] for k in ("achieved", "frac", "traffic", "b_alg_per_cell", "avg_kernel_us", "kernel")}}
+
+
+def measure(ctx, workload, K, W, chunk_cap, timing_steps, single_launch, n=None, tile=None, threads=0):
+    """One instance of `workload`: W warm-up steps, then K timed steps replayed
+    from a captured graph; returns the JSON line without cpu_baseline."""
+    w = WORKLOADS[workload]
+    p, fields = make(workload, n=n, tile=tile, threads=threads)
+    ncell = p.nx * p.ny * p.nz
+
+    chunk = graph_chunk_for(K, chunk_cap)
+    ctx.set_single_launch(single_launch)
+    inst = ctx.create(p)
+    for f, a in enumerate(fields):
+        inst.upload(f, a)
+    inst.step(W)
+    # Capture, instantiate and replay once the graph the timed region replays,
+    # whatever W is, so the timed region launches a cached, warm graph only.
+    inst.step(chunk, graph_chunk=chunk)
+    ctx.sync()
+    info0 = inst.info()
+    u0 = info0.cell_updates
+    single = bool(info0.single_launch)
+    ctx.marker(0)
+    t0 = time.perf_counter()
+    inst.step(K, graph_chunk=chunk)
+    ctx.marker(1)
+    ctx.sync()
+    t1 = time.perf_counter()
+    u1 = inst.info().cell_updates
+    gpu_ms = ctx.elapsed_ms(0, 1)
+    wall_s = t1 - t0
+    active_updates = u1 - u0
+    value = active_updates / wall_s
+    active_frac = active_updates / (K * ncell)
+
+    # kernel time of a long graph-replayed run (the host enqueues every replay
+    # before the first ends, so no launch latency sits between the two events)
+    reps = max(1, -(-timing_steps // chunk))
+    u4 = inst.info().cell_updates
+    ctx.marker(2)
+    inst.step(reps * chunk, graph_chunk=chunk)
+    ctx.marker(3)
+    ctx.sync()
+    u5 = inst.info().cell_updates
+    replay_ms = ctx.elapsed_ms(2, 3)
+    replay_steps = reps * chunk
+
+    # eager pass with hipEvents around every step kernel
+    TS = timing_steps
+    ctx.timing(True)
+    u2 = inst.info().cell_updates
+    inst.step(TS)
+    ctx.sync()
+    u3 = inst.info().cell_updates
+    k_ms, k_n, f_ms, f_n = ctx.timing_query()
+    ctx.timing(False)
+    finite = inst.check_finite()
+    inst.destroy()
+
+    eager_k_s = k_ms / 1e3 / max(1, k_n)
+    if single:
+        # one launch per step: the kernel's duration is the GPU time per step of
+        # a long graph-replayed run (hipEvents at its two ends; it also holds one
+        # state-restoring launch per graph chunk and the gaps between launches)
+        avg_k_s = replay_ms / 1e3 / replay_steps
+        cells_per_launch = (u5 - u4) / replay_steps
+        kernel = "single-launch step kernel (awake sets + fused stencil + detector)"
+    else:
+        # two launches per step: hipEvents around every step kernel in an eager pass
+        avg_k_s = eager_k_s
+        cells_per_launch = (u3 - u2) / max(1, k_n)
+        kernel = "step kernel (fused stencil + detector)"
+    bytes_per_launch = w["b_alg"] * cells_per_launch
+    achieved = bytes_per_launch / avg_k_s / 1e9
+    out = {
+        "metric": "active cell-updates/s (+ % HBM peak of the step kernel)",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": wall_s * 1e3 / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{workload}: {p.nx}x{p.ny}" + (f"x{p.nz}" if p.nz > 1 else "") +
+                        f" FP64, {p.tile_x}x{p.tile_y}" + (f"x{p.tile_z}" if p.nz > 1 else "") + f" tiles, {w['mode']}",
+            "grid": [p.nx, p.ny, p.nz], "tile": p.tile_x, "threads": p.threads or "default", "mode": w["mode"],
+            "graph_chunk": chunk,
+        },
+        "full_grid_updates_per_s": K * ncell / wall_s,
+        "active_fraction": active_frac,
+        "gpu_ms_per_step": gpu_ms / K,
+        "finite": bool(finite),
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None if (n or tile or threads) else traffic_from_profile(workload),
+            "kernel": kernel,
+            "b_alg_per_cell": w["b_alg"],
+            "avg_kernel_us": avg_k_s * 1e6,
+            "eager_kernel_us": eager_k_s * 1e6,
+            "avg_finalize_us": f_ms * 1e3 / max(1, f_n),
+            "launches_timed": K if single else k_n,
+        },
+    }
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -483,6 +618,8 @@ def main():
     ap.add_argument("--workload", default="ch", choices=sorted(WORKLOADS))
     ap.add_argument("--chunk", type=int, default=100, help="steps per captured hipGraph")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-companions", action="store_true",
+                    help="default workload: skip the other single-instance configurations")
     ap.add_argument("--timing-steps", type=int, default=500)
     ap.add_argument("--n", type=int, default=None, help="grid side (ch, ch_cons, dendrite, kh); default: the workload's")
     ap.add_argument("--tile", type=int, default=None, help="activation tile edge override")
@@ -516,113 +653,13 @@ def main():
     if args.workload == "copycal":
         print(json.dumps(copy_calibration()))
         return
-    p, fields = make(args.workload, n=args.n, tile=args.tile, threads=args.threads)
-    ncell = p.nx * p.ny * p.nz
-
-    chunk = graph_chunk_for(K, args.chunk)
     with Context(0) as ctx:
-        ctx.set_single_launch(args.single_launch)
-        inst = ctx.create(p)
-        for f, a in enumerate(fields):
-            inst.upload(f, a)
-        inst.step(W)
-        # Capture, instantiate and replay once the graph the timed region replays,
-        # whatever W is, so the timed region launches a cached, warm graph only.
-        inst.step(chunk, graph_chunk=chunk)
-        ctx.sync()
-        info0 = inst.info()
-        u0 = info0.cell_updates
-        single = bool(info0.single_launch)
-        ctx.marker(0)
-        t0 = time.perf_counter()
-        inst.step(K, graph_chunk=chunk)
-        ctx.marker(1)
-        ctx.sync()
-        t1 = time.perf_counter()
-        u1 = inst.info().cell_updates
-        gpu_ms = ctx.elapsed_ms(0, 1)
-        wall_s = t1 - t0
-        active_updates = u1 - u0
-        value = active_updates / wall_s
-        active_frac = active_updates / (K * ncell)
-
-        # kernel time of a long graph-replayed run (the host enqueues every replay
-        # before the first ends, so no launch latency sits between the two events)
-        reps = max(1, -(-args.timing_steps // chunk))
-        u4 = inst.info().cell_updates
-        ctx.marker(2)
-        inst.step(reps * chunk, graph_chunk=chunk)
-        ctx.marker(3)
-        ctx.sync()
-        u5 = inst.info().cell_updates
-        replay_ms = ctx.elapsed_ms(2, 3)
-        replay_steps = reps * chunk
-
-        # eager pass with hipEvents around every step kernel
-        TS = args.timing_steps
-        ctx.timing(True)
-        u2 = inst.info().cell_updates
-        inst.step(TS)
-        ctx.sync()
-        u3 = inst.info().cell_updates
-        k_ms, k_n, f_ms, f_n = ctx.timing_query()
-        ctx.timing(False)
-        finite = inst.check_finite()
-        inst.destroy()
-
-    eager_k_s = k_ms / 1e3 / max(1, k_n)
-    if single:
-        # one launch per step: the kernel's duration is the GPU time per step of
-        # a long graph-replayed run (hipEvents at its two ends; it also holds one
-        # state-restoring launch per graph chunk and the gaps between launches)
-        avg_k_s = replay_ms / 1e3 / replay_steps
-        cells_per_launch = (u5 - u4) / replay_steps
-        kernel = "single-launch step kernel (awake sets + fused stencil + detector)"
-    else:
-        # two launches per step: hipEvents around every step kernel in an eager pass
-        avg_k_s = eager_k_s
-        cells_per_launch = (u3 - u2) / max(1, k_n)
-        kernel = "step kernel (fused stencil + detector)"
-    bytes_per_launch = w["b_alg"] * cells_per_launch
-    achieved = bytes_per_launch / avg_k_s / 1e9
-    out = {
-        "metric": "active cell-updates/s (+ % HBM peak of the step kernel)",
-        "value": value,
-        "unit": "cell-updates/s",
-        "n_gpus": 1,
-        "steps": K,
-        "warmup": W,
-        "ms_per_step": wall_s * 1e3 / K,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {
-            "workload": f"{args.workload}: {p.nx}x{p.ny}" + (f"x{p.nz}" if p.nz > 1 else "") +
-                        f" FP64, {p.tile_x}x{p.tile_y}" + (f"x{p.tile_z}" if p.nz > 1 else "") + f" tiles, {w['mode']}",
-            "grid": [p.nx, p.ny, p.nz], "tile": p.tile_x, "threads": p.threads or "default", "mode": w["mode"],
-            "graph_chunk": chunk,
-        },
-        "full_grid_updates_per_s": K * ncell / wall_s,
-        "active_fraction": active_frac,
-        "gpu_ms_per_step": gpu_ms / K,
-        "finite": bool(finite),
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None if (args.n or args.tile or args.threads) else traffic_from_profile(args.workload),
-            "kernel": kernel,
-            "b_alg_per_cell": w["b_alg"],
-            "avg_kernel_us": avg_k_s * 1e6,
-            "eager_kernel_us": eager_k_s * 1e6,
-            "avg_finalize_us": f_ms * 1e3 / max(1, f_n),
-            "launches_timed": K if single else k_n,
-        },
-    }
+        out = measure(ctx, args.workload, K, W, args.chunk, args.timing_steps, args.single_launch,
+                      n=args.n, tile=args.tile, threads=args.threads)
+        if args.workload == "ch" and not (args.n or args.tile or args.threads or args.no_companions):
+            out["companions"] = [companion(measure(ctx, c, WORKLOADS[c]["steps"], WORKLOADS[c]["warm"], args.chunk,
+                                                   args.timing_steps, args.single_launch))
+                                 for c in COMPANIONS]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.workload, n=args.n)
     print(json.dumps(out))

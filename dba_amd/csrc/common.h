@@ -110,7 +110,8 @@ __device__ __forceinline__ int block_any(int pred, int* lds_scratch) {
 }
 
 // Workgroup-wide OR of two predicates in one exchange: bit 0 = any(a), bit 1 = any(b).
-template <int NT>
+// TAIL = false drops the closing barrier, for a last use of lds_scratch.
+template <int NT, bool TAIL = true>
 __device__ __forceinline__ int block_any2(int a, int b, int* lds_scratch) {
     constexpr int NW = NT / 64;
     const unsigned long long ba = __ballot(a), bb = __ballot(b);
@@ -120,7 +121,7 @@ __device__ __forceinline__ int block_any2(int a, int b, int* lds_scratch) {
     int r = 0;
 #pragma unroll
     for (int w = 0; w < NW; w++) r |= lds_scratch[w];
-    __syncthreads();
+    if (TAIL) __syncthreads();
     return r;
 }
 

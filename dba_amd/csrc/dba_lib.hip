@@ -103,6 +103,7 @@ struct Group {
     bool conserved = false;       // one conserved instance (single-launch steps carry its mean shift)
     bool partial = false;         // an instance with partial tiles: the kernels' PART instances
     int march = 0;                // column-march single-launch steps: tiles per strip (0: off)
+    bool spec = false;            // K93 single-launch steps with speculative staging
     bool nine = true;             // CH 9-point stencil (the march kernel is instanced on it)
     Inst* d_insts = nullptr;
     Seg* d_segs = nullptr;
@@ -475,6 +476,12 @@ dba_status get_group(dba_ctx* ctx, const std::vector<int>& ids, Group** out) {
                 break;
             }
     }
+    // Speculative staging in single-launch K93 steps (k93_dense_kernel SPEC):
+    // one instance of up to 1024 full 16 x 16 tiles at 512 threads (measured on
+    // 512^2: 7.2 -> 6.7 us per step with the run-counter prefetch); larger grids
+    // would pay the rings of both buffers of every sleeping tile in HBM bytes
+    g->spec = ids.size() == 1 && g->model == DBA_MODEL_DENDRITE_K93 && g->tile == 16 && g->dense_threads == 512 &&
+              first->h.full_tiles && first->h.nt <= 1024;
     // Model B: 65536 workgroups (4 blocks each on a fully awake 256^3 field)
     // measured faster than 16384 or 3072 with longer pipelines per wave
     int64_t cap = g->model == DBA_MODEL_MODELB ? 65536 : (g->model == DBA_MODEL_DENDRITE_K94 ? 16384 : 4096);
@@ -638,6 +645,9 @@ void launch_dense_tb(const Group* g, hipStream_t s, int first, int step) {
     } else if (g->partial) {
         hipLaunchKernelGGL((k93_dense_kernel<TB, NT, true>), grid, dim3(NT), 0, s, g->d_insts, g->d_dmap, first,
                            step);
+    } else if (TB == 16 && NT == 512 && g->spec) {
+        hipLaunchKernelGGL((k93_dense_kernel<16, 512, false, true>), grid, dim3(512), 0, s, g->d_insts, g->d_dmap,
+                           first, step);
     } else {
         hipLaunchKernelGGL((k93_dense_kernel<TB, NT, false>), grid, dim3(NT), 0, s, g->d_insts, g->d_dmap, first,
                            step);

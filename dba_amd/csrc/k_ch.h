@@ -360,6 +360,7 @@ __global__ __launch_bounds__(NT) void ch_dense_kernel(const Inst* __restrict__ i
     const int prev = s & 1, next = prev ^ 1;
     const bool regular = I.mode == 0;
     constexpr bool cons = CONS;  // conserved DBA: one instance in one segment (the host checks)
+    const int runs = threadIdx.x == 0 ? I.druns[t] : 0;  // read early, off the tail
     dense_nb_fill(nb, I, t, first, prev, regular, 1, cons);
     if (cons && !first) {
         // The previous step's change sums (tpart + (1 + parity) nt, 0 for tiles
@@ -406,7 +407,7 @@ __global__ __launch_bounds__(NT) void ch_dense_kernel(const Inst* __restrict__ i
         I.dF[next][t] = (uint8_t)any;
         I.dP[next][t] = (uint8_t)(pself ^ ran);
         if (ran) {
-            I.druns[t] += 1;
+            I.druns[t] = runs + 1;
             if (I.hist_cap > 0) {
                 const int64_t step = I.ctr[0] + s;
                 if (step < I.hist_cap) atomicAdd((int*)&I.hist[2 * step], 1);

@@ -153,7 +153,6 @@ struct FinLds {
     int s_wcnt[kSegWords];          // per 64-tile word: tiles in the next worklist
     unsigned long long s_wran[kSegWords][3];  // per word: tiles that ran (set 0, set 1), their cells
     double sd[kFinNT / 64 + 1];
-    int si[kFinNT / 64 + 2];
     double s_cs[kMaxFields];
     double s_scal[16];
     int64_t s_step;
@@ -174,7 +173,6 @@ __device__ __forceinline__ void finalize_body(const Inst* __restrict__ insts, co
     auto& s_wcnt = F.s_wcnt;
     auto& s_wran = F.s_wran;
     auto& sd = F.sd;
-    auto& si = F.si;
     auto& s_cs = F.s_cs;
     auto& s_scal = F.s_scal;
     auto& s_step = F.s_step;

@@ -57,10 +57,12 @@ struct K93Lds {
 // One tile's step (two sets: p, T). WORKLIST: fill the neighbour table from the
 // record and store the detector bytes here; otherwise the caller has filled the
 // table and stores the returned bits (p) | (T) << 1 (uniform over the
-// workgroup, 0 in regular mode; bit s is meaningful only for a set that ran).
+// workgroup, 0 in regular mode; bit s is meaningful only for a set that ran),
+// and, the workgroup's only tile, it skips the closing barriers.
 // PARTIAL: a partial last tile (w x h cells) when the grid is not a multiple of
 // the tile; the owner of each halo cell then comes from its mapped position.
-template <int TB, int NT, bool WORKLIST, bool PARTIAL = false>
+// STAGED: the caller has already written sp / sT (k93_stage_select).
+template <int TB, int NT, bool WORKLIST, bool PARTIAL = false, bool STAGED = false>
 __device__ __forceinline__ int k93_tile(CInst& I, int t, uint64_t rec, NbTable& nb, K93Lds<TB>& L,
                                         int* sflag) {
     constexpr int W2 = TB + 4;
@@ -96,6 +98,7 @@ __device__ __forceinline__ int k93_tile(CInst& I, int t, uint64_t rec, NbTable& 
 
     // stage phi (2-cell halo) when p-awake, T (1-cell halo) when T-awake,
     // and phi / T of the tile itself always (the T update needs phi' - phi)
+    if (!STAGED)
     for (int k = tid; k < (w + 4) * (h + 4); k += NT) {
         const int lx = k % (w + 4), ly = k / (w + 4);
         const bool own = lx >= 2 && lx < w + 2 && ly >= 2 && ly < h + 2;
@@ -104,6 +107,7 @@ __device__ __forceinline__ int k93_tile(CInst& I, int t, uint64_t rec, NbTable& 
         const int d = PARTIAL ? (own ? 4 : odir(mx, my)) : nb_dir1<TB, 2>(lx) + 3 * nb_dir1<TB, 2>(ly);
         sp[lx + ly * W2] = (nb.par[0][d] ? p1 : p0)[mx + (int64_t)my * nx];
     }
+    if (!STAGED)
     for (int k = tid; k < (w + 2) * (h + 2); k += NT) {
         const int lx = k % (w + 2), ly = k / (w + 2);
         const bool own = lx >= 1 && lx <= w && ly >= 1 && ly <= h;
@@ -162,14 +166,14 @@ __device__ __forceinline__ int k93_tile(CInst& I, int t, uint64_t rec, NbTable& 
     }
     int bits = 0;
     if (!regular) {
-        bits = block_any2<NT>(fp, fT, sflag);
+        bits = block_any2<NT, WORKLIST>(fp, fT, sflag);
         const int anyp = bits & 1, anyT = bits >> 1;
         if (WORKLIST && tid == 0) {
             if (ap) I.nflag[0][t] = (uint8_t)anyp;
             if (at) I.nflag[1][t] = (uint8_t)anyT;
         }
     }
-    __syncthreads();
+    if (WORKLIST) __syncthreads();
     return bits;
 }
 
@@ -200,10 +204,79 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k9
     }
 }
 
+// Speculative staging of a full tile (single-launch steps of small grids): the
+// phi ring (2 cells) and T ring (1 cell) of both buffers are loaded into
+// registers before the neighbour table is known, so the table's loads and these
+// overlap; k93_stage_select then keeps, per cell, the buffer the owner's parity
+// names. The extra bytes are the other buffer's copy of the rings of every
+// tile, sleeping ones included, which is why only grids of up to 1024 tiles
+// (all workgroups resident at once, the grid in the caches) take this path.
+template <int TB, int NT>
+struct K93Spec {
+    static constexpr int W2 = TB + 4, W1 = TB + 2;
+    static constexpr int KP = (W2 * W2 + NT - 1) / NT, KT = (W1 * W1 + NT - 1) / NT;
+    double p[2][KP], T[2][KT];
+};
+
+template <int TB, int NT>
+__device__ __forceinline__ void k93_stage_load(CInst& I, int t, K93Spec<TB, NT>& r) {
+    using S = K93Spec<TB, NT>;
+    const int ntx = I.ntx, nx = I.nx, ny = I.ny, bcx = I.bc_x, bcy = I.bc_y;
+    const int ti = t % ntx, tj = t / ntx, x0 = ti * TB, y0 = tj * TB;
+#pragma unroll
+    for (int q = 0; q < S::KP; q++) {
+        const int k = threadIdx.x + q * NT;
+        if (k < S::W2 * S::W2) {
+            const int lx = k % S::W2, ly = k / S::W2;
+            const int64_t gi = map_index(x0 + lx - 2, nx, bcx) + (int64_t)map_index(y0 + ly - 2, ny, bcy) * nx;
+            r.p[0][q] = I.buf[0][0][gi];
+            r.p[1][q] = I.buf[1][0][gi];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < S::KT; q++) {
+        const int k = threadIdx.x + q * NT;
+        if (k < S::W1 * S::W1) {
+            const int lx = k % S::W1, ly = k / S::W1;
+            const int64_t gi = map_index(x0 + lx - 1, nx, bcx) + (int64_t)map_index(y0 + ly - 1, ny, bcy) * nx;
+            r.T[0][q] = I.buf[0][1][gi];
+            r.T[1][q] = I.buf[1][1][gi];
+        }
+    }
+}
+
+// The staging of k93_tile from the speculative registers: phi of the tile
+// always and its ring when p-awake, T likewise when T-awake.
+template <int TB, int NT>
+__device__ __forceinline__ void k93_stage_select(const NbTable& nb, const K93Spec<TB, NT>& r, K93Lds<TB>& L) {
+    using S = K93Spec<TB, NT>;
+    const int ap = nb.act[0][4], at = nb.act[1][4];
+#pragma unroll
+    for (int q = 0; q < S::KP; q++) {
+        const int k = threadIdx.x + q * NT;
+        if (k < S::W2 * S::W2) {
+            const int lx = k % S::W2, ly = k / S::W2;
+            const bool own = lx >= 2 && lx < TB + 2 && ly >= 2 && ly < TB + 2;
+            if (own || ap) L.sp[k] = nb.par[0][nb_dir1<TB, 2>(lx) + 3 * nb_dir1<TB, 2>(ly)] ? r.p[1][q] : r.p[0][q];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < S::KT; q++) {
+        const int k = threadIdx.x + q * NT;
+        if (k < S::W1 * S::W1) {
+            const int lx = k % S::W1, ly = k / S::W1;
+            const bool own = lx >= 1 && lx <= TB && ly >= 1 && ly <= TB;
+            if (own || at) L.sT[k] = nb.par[1][nb_dir1<TB, 1>(lx) + 3 * nb_dir1<TB, 1>(ly)] ? r.T[1][q] : r.T[0][q];
+        }
+    }
+}
+
 // Single-launch step of a K93 group (see ch_dense_kernel): per
 // set, awake bits and parities of the 3x3 owner tiles from the bitmaps (first
 // step) or the previous step's bytes; the tile runs when either set is awake.
-template <int TB, int NT, bool PART>
+// SPEC: one instance of full tiles, staged speculatively (K93Spec). The tile's
+// run counter is read at the start, off the path of the tile's last store.
+template <int TB, int NT, bool PART, bool SPEC = false>
 __global__ __launch_bounds__(NT) void k93_dense_kernel(const Inst* __restrict__ insts,
                                                        const int32_t* __restrict__ dmap, int first, int s) {
     __shared__ K93Lds<TB> L;
@@ -214,20 +287,29 @@ __global__ __launch_bounds__(NT) void k93_dense_kernel(const Inst* __restrict__ 
     const int nt = I.nt;
     const int prev = s & 1, next = prev ^ 1;
     const bool regular = I.mode == 0;
+    const int runs = threadIdx.x == 0 ? I.druns[t] : 0;
+    K93Spec<TB, NT> spec;
+    if (SPEC) k93_stage_load<TB, NT>(I, t, spec);
     dense_nb_fill(nb, I, t, first, prev, regular, 2, false);
     __syncthreads();
     const int ap = nb.act[0][4], at = nb.act[1][4];
     int bits = 0;
-    if (ap | at)
-        bits = k93_partial<TB, PART>(I, t) ? k93_tile<TB, NT, false, true>(I, t, 0, nb, L, sflag)
-                                     : k93_tile<TB, NT, false>(I, t, 0, nb, L, sflag);
+    if (ap | at) {
+        if (SPEC) {
+            k93_stage_select<TB, NT>(nb, spec, L);
+            bits = k93_tile<TB, NT, false, false, true>(I, t, 0, nb, L, sflag);
+        } else {
+            bits = k93_partial<TB, PART>(I, t) ? k93_tile<TB, NT, false, true>(I, t, 0, nb, L, sflag)
+                                               : k93_tile<TB, NT, false>(I, t, 0, nb, L, sflag);
+        }
+    }
     if (threadIdx.x == 0) {
         I.dF[next][t] = (uint8_t)(ap ? bits & 1 : 0);
         I.dF[next][nt + t] = (uint8_t)(at ? bits >> 1 : 0);
         I.dP[next][t] = (uint8_t)(nb.par[0][4] ^ ap);
         I.dP[next][nt + t] = (uint8_t)(nb.par[1][4] ^ at);
         if (ap | at) {
-            I.druns[t] += 1 + ap;  // field-set updates: T over A_p | A_T, p over A_p
+            I.druns[t] = runs + 1 + ap;  // field-set updates: T over A_p | A_T, p over A_p
             if (I.hist_cap > 0) {
                 const int64_t step = I.ctr[0] + s;
                 if (step < I.hist_cap) {

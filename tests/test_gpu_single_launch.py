@@ -82,11 +82,14 @@ def k93_run(ctx, p, phi, T, single, plan):
     return out
 
 
-@pytest.mark.parametrize("tile,mode", [(16, abi.MODE_DBA), (8, abi.MODE_DBA), (32, abi.MODE_DBA),
-                                       (16, abi.MODE_REGULAR)])
-def test_k93_single_launch_equals_worklist(ctx, tile, mode):
+# 16 x 16 tiles at the default 512 threads take the speculative staging (both
+# buffers' rings loaded before the neighbour table); 8 and 32 do not
+@pytest.mark.parametrize("tile,mode,bc", [(16, abi.MODE_DBA, abi.BC_NOFLUX), (8, abi.MODE_DBA, abi.BC_NOFLUX),
+                                          (32, abi.MODE_DBA, abi.BC_NOFLUX), (16, abi.MODE_REGULAR, abi.BC_NOFLUX),
+                                          (16, abi.MODE_DBA, abi.BC_PERIODIC)])
+def test_k93_single_launch_equals_worklist(ctx, tile, mode, bc):
     n = 256
-    p = configs.dendrite_k93_params(n=n, tile=tile, mode=mode, history_cap=400)
+    p = configs.dendrite_k93_params(n=n, tile=tile, mode=mode, history_cap=400, bc=bc)
     phi, T = configs.dendrite_k93_ic(n, n, seeds=[(90, 100), (170, 150)])
     plan = [(1, 0), (9, 0), (150, 50), (33, 10), (2, 0)]
     a = k93_run(ctx, p, phi, T, False, plan)
