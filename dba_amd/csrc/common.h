@@ -78,6 +78,13 @@ struct Inst {
 // The descriptor as device code reads it (scalar loads).
 using CInst = const DBA_CONST Inst;
 
+// A per-tile counter read at the start of a workgroup and used at its end: a
+// vector load, so it waits only where the value is used (a uniform address
+// would otherwise become a scalar load that the wave waits for at once).
+__device__ __forceinline__ int load_early(const gi32* p) {
+    return __hip_atomic_load((int*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // Boundary index map: periodic wrap, or mirror about the boundary face.
 __device__ __forceinline__ int map_index(int i, int n, int bc) {
     if (bc == 0) {

@@ -465,12 +465,13 @@ dba_status get_group(dba_ctx* ctx, const std::vector<int>& ids, Group** out) {
     // Column-march single-launch steps (ch_march_kernel): one large CH instance of
     // full 32 x 32 tiles, periodic both ways, not conserved; K tiles per strip,
     // the most that keep >= 1024 strips (measured: 2048^2 K = 4 18.4 us against
-    // 20.8 at K = 2 and 22.5 at K = 8; 4096^2 K = 8 61.0 us against 63.1 at K = 4)
+    // 20.8 at K = 2 and 22.5 at K = 8; 4096^2 K = 16 59.7 us, K = 8 60.2 us, K = 4
+    // 63.1 us; 8192^2 K = 32 248.9 us against 253.0 at K = 8)
     g->nine = first->p.stencil != 5;
     if (ids.size() == 1 && g->model == DBA_MODEL_CH && g->tile == 32 && g->dense_threads == 256 &&
         first->p.mode != DBA_MODE_CONSERVED && first->h.full_tiles && first->p.bc_x == DBA_BC_PERIODIC &&
         first->p.bc_y == DBA_BC_PERIODIC && first->h.nt >= kMarchMinTiles) {
-        for (int k : {8, 4, 2})
+        for (int k : {32, 16, 8, 4, 2})
             if (first->h.nty % k == 0 && first->h.nt / k >= 1024) {
                 g->march = k;
                 break;
@@ -668,6 +669,8 @@ void launch_dense_nt(const Group* g, hipStream_t s, int first, int step) {
 void launch_dense(const Group* g, hipStream_t s, int first, int step) {
     if (g->march) {
         switch (g->march) {
+            case 32: launch_march<32>(g, s, first, step); break;
+            case 16: launch_march<16>(g, s, first, step); break;
             case 8: launch_march<8>(g, s, first, step); break;
             case 4: launch_march<4>(g, s, first, step); break;
             default: launch_march<2>(g, s, first, step); break;

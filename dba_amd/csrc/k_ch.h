@@ -360,7 +360,7 @@ __global__ __launch_bounds__(NT) void ch_dense_kernel(const Inst* __restrict__ i
     const int prev = s & 1, next = prev ^ 1;
     const bool regular = I.mode == 0;
     constexpr bool cons = CONS;  // conserved DBA: one instance in one segment (the host checks)
-    const int runs = threadIdx.x == 0 ? I.druns[t] : 0;  // read early, off the tail
+    const int runs = load_early(I.druns + t);  // read early, off the tail (every lane: no branch)
     dense_nb_fill(nb, I, t, first, prev, regular, 1, cons);
     if (cons && !first) {
         // The previous step's change sums (tpart + (1 + parity) nt, 0 for tiles

@@ -83,19 +83,16 @@ __device__ __forceinline__ Nb2 nb2_of(int ti, int tj, int ntx, int nty, int bcx,
     return n;
 }
 
-__device__ __forceinline__ int dilated2d(const uint8_t* N, int ti, int tj, int ntx, const Nb2& n, int kind) {
-    const int row = tj * ntx;
-    int r = N[ti + row];
-    if (n.xm >= 0) r |= N[n.xm + row];
-    if (n.xp >= 0) r |= N[n.xp + row];
-    if (n.ym >= 0) r |= N[ti + n.ym * ntx];
-    if (n.yp >= 0) r |= N[ti + n.yp * ntx];
-    if (kind == 1) {
-        if (n.ym >= 0 && n.xm >= 0) r |= N[n.xm + n.ym * ntx];
-        if (n.ym >= 0 && n.xp >= 0) r |= N[n.xp + n.ym * ntx];
-        if (n.yp >= 0 && n.xm >= 0) r |= N[n.xm + n.yp * ntx];
-        if (n.yp >= 0 && n.xp >= 0) r |= N[n.xp + n.yp * ntx];
-    }
+__device__ __forceinline__ int dilated2d(const gu8* N, int ti, int tj, int ntx, const Nb2& n, int kind) {
+    // A missing neighbour (zero-flux or wall boundary) reads the tile itself
+    // instead, which the OR already holds (a missing corner falls back to the
+    // edge neighbour that exists, or to the tile): no load depends on a branch,
+    // so all of them are in flight at once instead of one after another.
+    const int xm = n.xm >= 0 ? n.xm : ti, xp = n.xp >= 0 ? n.xp : ti;
+    const int ym = n.ym >= 0 ? n.ym : tj, yp = n.yp >= 0 ? n.yp : tj;
+    const int row = tj * ntx, rm = ym * ntx, rp = yp * ntx;
+    int r = N[ti + row] | N[xm + row] | N[xp + row] | N[ti + rm] | N[ti + rp];
+    if (kind == 1) r |= N[xm + rm] | N[xp + rm] | N[xm + rp] | N[xp + rp];
     return r != 0;
 }
 
@@ -470,24 +467,37 @@ __device__ __forceinline__ void dense_nb_fill(NbTable& nb, CInst& I, int t, int 
     const int o = nb_owner2d(I, ti, tj, d);
     const int oj = o / ntx, oi = o - oj * ntx;
     nb.tile[d] = o;
-    // conserved: both shift epochs of the owner, loaded beside its parity
+    // every load first (both sets, the conserved epochs), then the table
     const double e0 = cons ? I.s_tile[o] : 0.0, e1 = cons ? I.s_tile[nt + o] : 0.0;
+    int par[2] = {0, 0}, act[2] = {1, 1};
+    if (first) {
+        for (int s = 0; s < nsets; s++) par[s] = bit_of(I.par[s], o);
+        if (!regular)
+            for (int s = 0; s < nsets; s++) act[s] = bit_of(I.a_cur[s], o);
+    } else {
+        const Nb2 n = nb2_of(oi, oj, ntx, I.nty, I.bc_x, I.bc_y);
+        // the four per-tile byte arrays' pointers in one batch of scalar loads,
+        // then a select (the empty asm keeps the compiler from turning the select
+        // of two loaded pointers back into a load from a selected address, which
+        // put a dependent scalar load in front of each array's bytes)
+        const gu8* F0 = I.dF[0];
+        const gu8* F1 = I.dF[1];
+        const gu8* P0 = I.dP[0];
+        const gu8* P1 = I.dP[1];
+        asm volatile("" : "+s"(F0), "+s"(F1), "+s"(P0), "+s"(P1));
+        const gu8* F = prev ? F1 : F0;
+        const gu8* P = prev ? P1 : P0;
+        for (int s = 0; s < nsets; s++) par[s] = P[s * nt + o];
+        if (!regular)
+            for (int s = 0; s < nsets; s++) act[s] = dilated2d(F + s * nt, oi, oj, ntx, n, 0);
+    }
     for (int s = 0; s < nsets; s++) {
-        int par, act;
-        if (first) {
-            par = bit_of(I.par[s], o);
-            act = regular ? 1 : bit_of(I.a_cur[s], o);
-        } else {
-            par = I.dP[prev][s * nt + o];
-            act = regular ? 1
-                          : dilated2d(I.dF[prev] + s * nt, oi, oj, ntx, nb2_of(oi, oj, ntx, I.nty, I.bc_x, I.bc_y), 0);
-        }
-        nb.par[s][d] = par;
-        nb.act[s][d] = act;
+        nb.par[s][d] = par[s];
+        nb.act[s][d] = act[s];
     }
     // conserved: the epoch of the owner's current buffer (the caller subtracts
     // it from the step's cumulative shift once that is known)
-    for (int f = 0; f < kMaxFields; f++) nb.shift[f][d] = (cons && f == 0) ? (nb.par[0][d] ? e1 : e0) : 0.0;
+    for (int f = 0; f < kMaxFields; f++) nb.shift[f][d] = (cons && f == 0) ? (par[0] ? e1 : e0) : 0.0;
 }
 
 // End of a run of n >= 1 single-launch steps: the state the worklist path and

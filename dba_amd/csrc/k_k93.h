@@ -287,7 +287,7 @@ __global__ __launch_bounds__(NT) void k93_dense_kernel(const Inst* __restrict__ 
     const int nt = I.nt;
     const int prev = s & 1, next = prev ^ 1;
     const bool regular = I.mode == 0;
-    const int runs = threadIdx.x == 0 ? I.druns[t] : 0;
+    const int runs = load_early(I.druns + t);  // every lane: no branch to wait at
     K93Spec<TB, NT> spec;
     if (SPEC) k93_stage_load<TB, NT>(I, t, spec);
     dense_nb_fill(nb, I, t, first, prev, regular, 2, false);

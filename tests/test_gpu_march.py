@@ -37,9 +37,11 @@ def run(ctx, p, c0, steps, mode, chunk):
 
 
 @pytest.mark.parametrize("mode,stencil,nx,ny", [(abi.MODE_DBA, 9, 4096, 2048), (abi.MODE_DBA, 5, 4096, 2048),
-                                                (abi.MODE_REGULAR, 9, 4096, 2048), (abi.MODE_DBA, 9, 2048, 2048)])
+                                                (abi.MODE_REGULAR, 9, 4096, 2048), (abi.MODE_DBA, 9, 2048, 2048),
+                                                (abi.MODE_DBA, 9, 4096, 4096), (abi.MODE_DBA, 5, 8192, 4096)])
 def test_march_equals_worklist(ctx, mode, stencil, nx, ny):
-    # 4096 x 2048: 128 x 64 tiles, strips of 8 tiles; 2048^2: strips of 4
+    # strips of 8 tiles on 4096 x 2048 (128 x 64 tiles), 4 on 2048^2, 16 on 4096^2,
+    # 32 on 8192 x 4096: the most tiles per strip that keep >= 1024 strips
     p = configs.ch_params(n=nx, ny=ny, tile=32, mode=mode, stencil=stencil, eps=2e-3, react_gamma=0.01,
                           cbar=0.05, history_cap=64)
     c0 = local_ic(nx, ny)
