@@ -357,6 +357,7 @@ __global__ __launch_bounds__(NT) void ch_dense_kernel(const Inst* __restrict__ i
     __shared__ double sW[kFinNT / 64];
     int t;
     CInst& I = dense_item(insts, dmap, &t);
+    dense_prefetch(I, 1);
     const int prev = s & 1, next = prev ^ 1;
     const bool regular = I.mode == 0;
     constexpr bool cons = CONS;  // conserved DBA: one instance in one segment (the host checks)
@@ -483,6 +484,7 @@ __global__ __launch_bounds__(256) void ch_march_kernel(const Inst* __restrict__ 
     constexpr int W2 = TB + 4, W1 = TB + 2;
     constexpr int NH1 = W1 * W1 - TB * TB;
     constexpr int NC = 3 * (K + 2);  // columns (left, own, right) x tile rows ty0 - 1 .. ty0 + K
+    static_assert(NC <= 128 && K <= NT - 128, "table threads below 128, run-count threads from 128");
     constexpr int MASK = kMarchRing - 1;
     __shared__ double ring[kMarchRing * W2];
     __shared__ double smu[W1 * W1];
@@ -490,6 +492,7 @@ __global__ __launch_bounds__(256) void ch_march_kernel(const Inst* __restrict__ 
     __shared__ int s_runs[K];
     __shared__ int sflag[NT / 64 + 1];
     CInst& I = ((CInst*)insts)[0];
+    dense_prefetch(I, 1);
     const int tid = threadIdx.x, tx = tid % TB, ty = tid / TB;
     const int nx = I.nx, ny = I.ny, ntx = I.ntx, nty = I.nty;
     const int prev = s & 1, next = prev ^ 1;
@@ -511,20 +514,23 @@ __global__ __launch_bounds__(256) void ch_march_kernel(const Inst* __restrict__ 
         const int oj0 = ty0 + r - 1;
         const int oj = oj0 < 0 ? oj0 + nty : (oj0 >= nty ? oj0 - nty : oj0);
         const int o = oi + oj * ntx;
-        int par, act;
+        int par, act = 1;
         if (first) {
             par = bit_of(I.par[0], o);
-            act = regular ? 1 : bit_of(I.a_cur[0], o);
+            if (!regular) act = bit_of(I.a_cur[0], o);
         } else {
-            par = I.dP[prev][o];
-            act = regular ? 1 : dilated2d(I.dF[prev], oi, oj, ntx, nb2_of(oi, oj, ntx, nty, 0, 0), 0);
+            const gu8 *F, *P;
+            dense_bytes(I, prev, &F, &P);
+            par = P[o];
+            if (!regular) act = dilated2d(F, oi, oj, ntx, nb2_of(oi, oj, ntx, nty, 0, 0), 0);
         }
         s_tile[tid] = o;
         s_par[tid] = par;
         s_act[tid] = act;
-    } else if (tid < NC + K) {
+    } else if (tid >= 128 && tid < 128 + K) {
         // the strip's per-tile update counts, read now so no tile waits on them
-        s_runs[tid - NC] = I.druns[sx + (ty0 + tid - NC) * ntx];
+        // (a wave of their own, so they load alongside the table)
+        s_runs[tid - 128] = I.druns[sx + (ty0 + tid - 128) * ntx];
     }
     __syncthreads();
 

@@ -37,35 +37,23 @@ __device__ __forceinline__ int64_t tile_cells(CInst& I, int t) {
 
 // Awake flag of tile t for the next step: OR of the detector bytes over the
 // tile and its neighbours (kind 0: faces; kind 1: 3x3 in-plane + z faces),
-// wrapping where the instance is periodic and clamping elsewhere.
-__device__ __forceinline__ int dilated3d(CInst& I, const uint8_t* N, int t, int kind) {
-    const int ti = t % I.ntx, tj = (t / I.ntx) % I.nty, tk = t / (I.ntx * I.nty);
-    int r = 0;
-    for (int dk = -1; dk <= 1; dk++) {
-        int kk = tk + dk;
-        if (kk < 0 || kk >= I.ntz) {
-            if (I.bc_z != 0) continue;
-            kk = (kk + I.ntz) % I.ntz;
-        }
-        for (int dj = -1; dj <= 1; dj++) {
-            int jj = tj + dj;
-            if (jj < 0 || jj >= I.nty) {
-                if (I.bc_y != 0) continue;
-                jj = (jj + I.nty) % I.nty;
-            }
-            for (int di = -1; di <= 1; di++) {
-                const int nzc = (di != 0) + (dj != 0) + (dk != 0);
-                if (kind == 0 && nzc > 1) continue;
-                if (kind == 1 && dk != 0 && (di != 0 || dj != 0)) continue;
-                int ii = ti + di;
-                if (ii < 0 || ii >= I.ntx) {
-                    if (I.bc_x != 0) continue;
-                    ii = (ii + I.ntx) % I.ntx;
-                }
-                r |= N[ii + jj * I.ntx + kk * I.ntx * I.nty];
-            }
-        }
-    }
+// wrapping where the instance is periodic and clamping elsewhere. A neighbour
+// clamped away reads the tile itself (or, for an in-plane corner, the edge
+// neighbour that exists), which the OR already holds: no load waits on a branch.
+__device__ __forceinline__ int dilated3d(CInst& I, const gu8* N, int t, int kind) {
+    const int ntx = I.ntx, nty = I.nty, ntz = I.ntz, nxy = ntx * nty;
+    const int ti = t % ntx, tj = (t / ntx) % nty, tk = t / nxy;
+    auto nb = [](int i, int d, int n, int bc) {
+        const int j = i + d;
+        return (j >= 0 && j < n) ? j : (bc == 0 ? (j + n) % n : i);
+    };
+    const int xm = nb(ti, -1, ntx, I.bc_x), xp = nb(ti, 1, ntx, I.bc_x);
+    const int ym = nb(tj, -1, nty, I.bc_y) * ntx, yp = nb(tj, 1, nty, I.bc_y) * ntx;
+    const int zm = nb(tk, -1, ntz, I.bc_z) * nxy, zp = nb(tk, 1, ntz, I.bc_z) * nxy;
+    const int row = tj * ntx, pl = tk * nxy;
+    int r = N[t] | N[xm + row + pl] | N[xp + row + pl] | N[ti + ym + pl] | N[ti + yp + pl] | N[ti + row + zm] |
+            N[ti + row + zp];
+    if (kind == 1) r |= N[xm + ym + pl] | N[xp + ym + pl] | N[xm + yp + pl] | N[xp + yp + pl];
     return r != 0;
 }
 
@@ -454,6 +442,35 @@ __device__ __forceinline__ CInst& dense_item(const Inst* insts, const int32_t* d
     return ((CInst*)insts)[d >> 24];
 }
 
+// The detector and parity bytes a single-launch step reads (step parity prev):
+// the four arrays' pointers in one batch of scalar loads, then a select. The
+// empty asm keeps the compiler from turning the select of two loaded pointers
+// back into a load from a selected address, which put a dependent scalar load
+// in front of each array's bytes.
+__device__ __forceinline__ void dense_bytes(CInst& I, int prev, const gu8** F, const gu8** P) {
+    const gu8* F0 = I.dF[0];
+    const gu8* F1 = I.dF[1];
+    const gu8* P0 = I.dP[0];
+    const gu8* P1 = I.dP[1];
+    asm volatile("" : "+s"(F0), "+s"(F1), "+s"(P0), "+s"(P1));
+    *F = prev ? F1 : F0;
+    *P = prev ? P1 : P0;
+}
+
+// The descriptor fields a single-launch step reads before its first barrier,
+// as one batch of scalar loads at the start of the workgroup: the empty asm
+// needs every value at that point, so the loads issue together and cost one
+// round trip, where the compiler had placed them one dependent wait after another.
+__device__ __forceinline__ void dense_prefetch(CInst& I, int nfield) {
+    const int nt = I.nt, ntx = I.ntx, nty = I.nty, nx = I.nx, ny = I.ny, bcx = I.bc_x, bcy = I.bc_y, mode = I.mode;
+    const gu8 *f0 = I.dF[0], *f1 = I.dF[1], *p0 = I.dP[0], *p1 = I.dP[1];
+    const gi32* runs = I.druns;
+    const gdouble *b00 = I.buf[0][0], *b10 = I.buf[1][0];
+    const gdouble *b01 = nfield > 1 ? I.buf[0][1] : b00, *b11 = nfield > 1 ? I.buf[1][1] : b10;
+    asm volatile("" ::"s"(nt), "s"(ntx), "s"(nty), "s"(nx), "s"(ny), "s"(bcx), "s"(bcy), "s"(mode), "s"(f0), "s"(f1),
+                 "s"(p0), "s"(p1), "s"(runs), "s"(b00), "s"(b10), "s"(b01), "s"(b11));
+}
+
 // Neighbour table of a single-launch step (threads 0..8; caller synchronises):
 // per set, buffer parity and awake bit of each 3x3 owner tile, from the bitmaps
 // at the first step of a run, afterwards from the previous step's per-tile bytes
@@ -476,17 +493,8 @@ __device__ __forceinline__ void dense_nb_fill(NbTable& nb, CInst& I, int t, int 
             for (int s = 0; s < nsets; s++) act[s] = bit_of(I.a_cur[s], o);
     } else {
         const Nb2 n = nb2_of(oi, oj, ntx, I.nty, I.bc_x, I.bc_y);
-        // the four per-tile byte arrays' pointers in one batch of scalar loads,
-        // then a select (the empty asm keeps the compiler from turning the select
-        // of two loaded pointers back into a load from a selected address, which
-        // put a dependent scalar load in front of each array's bytes)
-        const gu8* F0 = I.dF[0];
-        const gu8* F1 = I.dF[1];
-        const gu8* P0 = I.dP[0];
-        const gu8* P1 = I.dP[1];
-        asm volatile("" : "+s"(F0), "+s"(F1), "+s"(P0), "+s"(P1));
-        const gu8* F = prev ? F1 : F0;
-        const gu8* P = prev ? P1 : P0;
+        const gu8 *F, *P;
+        dense_bytes(I, prev, &F, &P);
         for (int s = 0; s < nsets; s++) par[s] = P[s * nt + o];
         if (!regular)
             for (int s = 0; s < nsets; s++) act[s] = dilated2d(F + s * nt, oi, oj, ntx, n, 0);

@@ -284,6 +284,7 @@ __global__ __launch_bounds__(NT) void k93_dense_kernel(const Inst* __restrict__ 
     __shared__ NbTable nb;
     int t;
     CInst& I = dense_item(insts, dmap, &t, false);
+    dense_prefetch(I, 2);
     const int nt = I.nt;
     const int prev = s & 1, next = prev ^ 1;
     const bool regular = I.mode == 0;
