@@ -178,15 +178,72 @@ __device__ __noinline__ int ch_tile_partial(CInst& I, int t, uint64_t rec, NbTab
     return any;
 }
 
+// Staging of a full tile for ch_tile: the thread's strip of c into registers
+// (cout) and LDS, the 2-cell halo into LDS, from the buffers the neighbour
+// table names, less the conserved shifts (the arithmetic of ch_tile's own
+// staging). Every load is issued before the first wait: a halo slot past the
+// region's count loads halo cell 0 again and stores nothing. (ch_tile's loop
+// had waited for each load in turn, and a branch on the stencil between the
+// strip's first load and the rest split them too.)
+template <int TB, int NT>
+__device__ __forceinline__ void ch_stage(CInst& I, int t, const NbTable& nb, double* sc, double* cout) {
+    constexpr int W2 = TB + 4;
+    constexpr int R = TB * TB / NT;
+    constexpr int NH2 = W2 * W2 - TB * TB;
+    constexpr int HK = (NH2 + NT - 1) / NT;
+    const int tid = threadIdx.x, tx = tid % TB, ty = tid / TB;
+    const int nx = I.nx, ny = I.ny, ntx = I.ntx, bcx = I.bc_x, bcy = I.bc_y;
+    const int x0 = (t % ntx) * TB, y0 = (t / ntx) * TB;
+    const gdouble* __restrict__ b0 = I.buf[0][0];
+    const gdouble* __restrict__ b1 = I.buf[1][0];
+    const gdouble* __restrict__ cur = nb.par[0][4] ? b1 : b0;
+    const int gx = x0 + tx;
+    double c[R], hv[HK], hs[HK];
+    int hl[HK];
+#pragma unroll
+    for (int r = 0; r < R; r++) c[r] = cur[gx + (int64_t)(y0 + ty * R + r) * nx];
+#pragma unroll
+    for (int q = 0; q < HK; q++) {
+        const int k0 = tid + q * NT;
+        const int k = k0 < NH2 ? k0 : 0;
+        int lx, ly;
+        if (k < 2 * W2) { ly = k / W2; lx = k % W2; }                      // two rows below
+        else if (k < 4 * W2) { ly = TB + 2 + (k - 2 * W2) / W2; lx = (k - 2 * W2) % W2; }  // two rows above
+        else { const int q4 = k - 4 * W2; ly = 2 + q4 / 4; const int c4 = q4 % 4; lx = c4 < 2 ? c4 : TB + c4; }
+        const int d = nb_dir1<TB, 2>(lx) + 3 * nb_dir1<TB, 2>(ly);
+        const int mx = map_index(x0 + lx - 2, nx, bcx), my = map_index(y0 + ly - 2, ny, bcy);
+        hv[q] = (nb.par[0][d] ? b1 : b0)[mx + (int64_t)my * nx];
+        hs[q] = nb.shift[0][d];
+        hl[q] = k0 < NH2 ? lx + ly * W2 : -1;
+    }
+    const double sh_self = nb.shift[0][4];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        cout[r] = c[r] - sh_self;
+        sc[(tx + 2) + (ty * R + r + 2) * W2] = cout[r];
+    }
+#pragma unroll
+    for (int q = 0; q < HK; q++)
+        if (hl[q] >= 0) sc[hl[q]] = hv[q] - hs[q];
+}
+
+// Whether tile t of I is a partial last tile (only in the kernels' PART instances).
+template <int TB, bool PART>
+__device__ __forceinline__ bool ch_partial(CInst& I, int t) {
+    return PART && !I.full_tiles && ((t % I.ntx) * TB + TB > I.nx || (t / I.ntx) * TB + TB > I.ny);
+}
+
 // One tile's step. WORKLIST: fill the neighbour table from the record and store
 // the detector byte here; otherwise the caller has filled the table (then a
 // barrier) and stores the returned detector bit any(|lap c| > eps) (0 in regular
 // mode, uniform over the workgroup) itself.
 // PART: the kernel serves instances with partial tiles (a separate instance of
 // the kernels, so the partial path's registers do not cost full-tile grids).
-template <int TB, int NT, bool NINE, bool WORKLIST, bool PART = false>
+// STAGED: the caller has filled the table (also in WORKLIST mode) and staged c
+// with ch_stage; cin holds the thread's strip.
+template <int TB, int NT, bool NINE, bool WORKLIST, bool PART = false, bool STAGED = false>
 __device__ __forceinline__ int ch_tile(CInst& I, int t, uint64_t rec, NbTable& nb, double* sc, double* smu,
-                                       double* sred, int* sflag) {
+                                       double* sred, int* sflag, const double* cin = nullptr) {
     constexpr int W2 = TB + 4;
     constexpr int W1 = TB + 2;
     constexpr int R = TB * TB / NT;
@@ -196,7 +253,7 @@ __device__ __forceinline__ int ch_tile(CInst& I, int t, uint64_t rec, NbTable& n
     const int tx = tid % TB, ty = tid / TB;
     const int nx = I.nx, ny = I.ny, ntx = I.ntx, bcx = I.bc_x, bcy = I.bc_y;
     const int x0 = (t % ntx) * TB, y0 = (t / ntx) * TB;
-    if (PART && !I.full_tiles && (x0 + TB > nx || y0 + TB > ny))
+    if (!STAGED && PART && !I.full_tiles && (x0 + TB > nx || y0 + TB > ny))
         return ch_tile_partial<TB, NT, NINE, WORKLIST>(I, t, rec, nb, sc, smu, sred, sflag);
     const bool regular = I.mode == 0;
     const bool cons = I.mode == 2;
@@ -205,7 +262,7 @@ __device__ __forceinline__ int ch_tile(CInst& I, int t, uint64_t rec, NbTable& n
     gdouble* __restrict__ mu_store = I.aux[0];
     const double kappa = I.kappa, eps = I.thresh_a;
     const double inv = NINE ? I.inv6dx2 : I.invdx2;
-    if (WORKLIST) {
+    if (WORKLIST && !STAGED) {
         nb_fill<TB>(nb, I, t, x0, y0, regular, 1, cons ? 1 : 0, rec);
         __syncthreads();
     }
@@ -217,11 +274,17 @@ __device__ __forceinline__ int ch_tile(CInst& I, int t, uint64_t rec, NbTable& n
     // 1. c: the thread's strip from HBM into registers and LDS, then the halo
     double c[R];
     const int gx = x0 + tx;
+    if (STAGED) {
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-        c[r] = cur[gx + (int64_t)(y0 + ty * R + r) * nx] - sh_self;
-        sc[(tx + 2) + (ty * R + r + 2) * W2] = c[r];
+        for (int r = 0; r < R; r++) c[r] = cin[r];
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            c[r] = cur[gx + (int64_t)(y0 + ty * R + r) * nx] - sh_self;
+            sc[(tx + 2) + (ty * R + r + 2) * W2] = c[r];
+        }
     }
+    if (!STAGED)
     for (int k = tid; k < NH2; k += NT) {
         int lx, ly;
         if (k < 2 * W2) { ly = k / W2; lx = k % W2; }                      // two rows below
@@ -330,8 +393,17 @@ __global__ __launch_bounds__(NT) void ch_step_kernel(const WorkList wl) {
         uint64_t rec;
         if (!work_next(wl, cur, total, &Ip, &t, &rec)) break;
         CInst& I = *Ip;
-        if (I.stencil != 5) ch_tile<TB, NT, true, true, PART>(I, t, rec, nb, sc, smu, sred, sflag);
-        else ch_tile<TB, NT, false, true, PART>(I, t, rec, nb, sc, smu, sred, sflag);
+        if (ch_partial<TB, PART>(I, t)) {
+            if (I.stencil != 5) ch_tile<TB, NT, true, true, PART>(I, t, rec, nb, sc, smu, sred, sflag);
+            else ch_tile<TB, NT, false, true, PART>(I, t, rec, nb, sc, smu, sred, sflag);
+            continue;
+        }
+        nb_fill<TB>(nb, I, t, 0, 0, I.mode == 0, 1, I.mode == 2 ? 1 : 0, rec);
+        __syncthreads();
+        double c[TB * TB / NT];
+        ch_stage<TB, NT>(I, t, nb, sc, c);
+        if (I.stencil != 5) ch_tile<TB, NT, true, true, PART, true>(I, t, rec, nb, sc, smu, sred, sflag, c);
+        else ch_tile<TB, NT, false, true, PART, true>(I, t, rec, nb, sc, smu, sred, sflag, c);
     }
 }
 
@@ -401,9 +473,17 @@ __global__ __launch_bounds__(NT) void ch_dense_kernel(const Inst* __restrict__ i
     }
     const int pself = nb.par[0][4], ran = nb.act[0][4];
     int any = 0;
-    if (ran)
-        any = I.stencil != 5 ? ch_tile<TB, NT, true, false, PART>(I, t, 0, nb, sc, smu, sred, sflag)
-                             : ch_tile<TB, NT, false, false, PART>(I, t, 0, nb, sc, smu, sred, sflag);
+    if (ran) {
+        if (ch_partial<TB, PART>(I, t)) {
+            any = I.stencil != 5 ? ch_tile<TB, NT, true, false, PART>(I, t, 0, nb, sc, smu, sred, sflag)
+                                 : ch_tile<TB, NT, false, false, PART>(I, t, 0, nb, sc, smu, sred, sflag);
+        } else {
+            double c[TB * TB / NT];
+            ch_stage<TB, NT>(I, t, nb, sc, c);
+            any = I.stencil != 5 ? ch_tile<TB, NT, true, false, PART, true>(I, t, 0, nb, sc, smu, sred, sflag, c)
+                                 : ch_tile<TB, NT, false, false, PART, true>(I, t, 0, nb, sc, smu, sred, sflag, c);
+        }
+    }
     if (threadIdx.x == 0) {
         I.dF[next][t] = (uint8_t)any;
         I.dP[next][t] = (uint8_t)(pself ^ ran);

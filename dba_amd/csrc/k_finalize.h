@@ -526,18 +526,27 @@ __device__ __forceinline__ void dense_canon_seg(CInst& I, int t0, int ntile, int
     const int last = n & 1;
     const int nsets = I.nsets;
     const int nw = (ntile + 63) >> 6;
-    const int r = valid ? I.druns[t] : 0;
+    // every load first, unconditionally (a thread past the segment reads tile t0,
+    // a one-set instance reads set 0 twice; neither result is used), so they are
+    // in flight together instead of waited for one branch at a time
+    const int tc = valid ? t : t0;
+    const int tj = tc / ntx, ti = tc - tj * ntx;
+    const Nb2 nbr = nb2_of(ti, tj, ntx, I.nty, I.bc_x, I.bc_y);
+    const gu8 *F, *P;
+    dense_bytes(I, last, &F, &P);
+    const int r0 = I.druns[tc];
+    int fv[2], pv[2], av[2];
+    for (int s = 0; s < 2; s++) {
+        const int so = s < nsets ? s * nt : 0;
+        fv[s] = F[so + tc];
+        pv[s] = P[so + tc];
+        av[s] = regular ? 1 : dilated2d(F + so, ti, tj, ntx, nbr, 0);
+    }
+    const int r = valid ? r0 : 0;
     uint64_t wa = 0ull;
     for (int s = 0; s < nsets; s++) {
-        int f = 0, p = 0, a = 0;
-        if (valid) {
-            f = I.dF[last][s * nt + t];
-            p = I.dP[last][s * nt + t];
-            const int tj = t / ntx, ti = t - tj * ntx;
-            a = regular ? 1
-                        : dilated2d(I.dF[last] + s * nt, ti, tj, ntx, nb2_of(ti, tj, ntx, I.nty, I.bc_x, I.bc_y), 0);
-            I.nflag[s][t] = (uint8_t)f;
-        }
+        const int f = valid ? fv[s] : 0, p = valid ? pv[s] : 0, a = valid ? av[s] : 0;
+        if (valid) I.nflag[s][t] = (uint8_t)f;
         const uint64_t ws = __ballot(a), wp = __ballot(p);
         wa |= ws;
         if (lane == 0 && wid < nw) {
