@@ -35,12 +35,24 @@ __device__ __forceinline__ void nb_fill3(NbTable3& nb, CInst& I, int x0, int y0,
     const int mx = map_index(rx, I.nx, 0), my = map_index(ry, I.ny, 0), mz = map_index(rz, I.nz, 0);
     const int tt = mx / TB + (my / TB) * I.ntx + (mz / TZ) * I.ntx * I.nty;
     nb.tile[d] = tt;
-    for (int s = 0; s < nsets; s++) {
-        nb.par[s][d] = bit_of(I.par[s], tt);
-        nb.act[s][d] = regular ? 1 : bit_of(I.a_cur[s], tt);
+    // both sets' bitmap words loaded together (a one-set instance reads set 0
+    // twice), from pointers selected out of one scalar batch, so no load waits
+    // on a branch or on a pointer load of its own
+    const gu64 *par0 = I.par[0], *par1 = I.par[1], *cur0 = I.a_cur[0], *cur1 = I.a_cur[1];
+    asm volatile("" : "+s"(par0), "+s"(par1), "+s"(cur0), "+s"(cur1));
+    const bool two = nsets > 1;
+    const uint64_t wp0 = par0[tt >> 6], wp1 = (two ? par1 : par0)[tt >> 6];
+    const uint64_t wa0 = cur0[tt >> 6], wa1 = (two ? cur1 : cur0)[tt >> 6];
+    const int b = tt & 63;
+    nb.par[0][d] = (int)((wp0 >> b) & 1ull);
+    nb.act[0][d] = regular ? 1 : (int)((wa0 >> b) & 1ull);
+    if (two) {
+        nb.par[1][d] = (int)((wp1 >> b) & 1ull);
+        nb.act[1][d] = regular ? 1 : (int)((wa1 >> b) & 1ull);
     }
+    const int p0 = (int)((wp0 >> b) & 1ull);
     for (int f = 0; f < kMaxFields; f++)
-        nb.shift[f][d] = f < ncons ? (I.scal[8 + f] - I.s_tile[((int64_t)f * 2 + nb.par[0][d]) * I.nt + tt]) : 0.0;
+        nb.shift[f][d] = f < ncons ? (I.scal[8 + f] - I.s_tile[((int64_t)f * 2 + p0) * I.nt + tt]) : 0.0;
 }
 
 // 3D wrap of a small periodic offset
@@ -549,13 +561,34 @@ __global__ __launch_bounds__(64) void k94_step_kernel(const WorkList wl) {
         __syncthreads();
         const int ap = nb.act[0][13], at = nb.act[1][13];
         const int pp0 = nb.par[0][13], pp1 = nb.par[1][13];
-        for (int k = tid; k < A; k += NT) {
-            const int lx = k % WX, ly = (k / WX) % WX, lz = k / (WX * WX);
-            const int d = nb_dir3<TB, TZ, 1>(lx, ly, lz);
-            const int64_t gi = wrapi(x0 + lx - 1, nx) + wrapi(y0 + ly - 1, ny) * (int64_t)nx +
-                               wrapi(z0 + lz - 1, nz) * nxy;
-            if (d == 13 || ap) sp[k] = I.buf[nb.par[0][d]][0][gi];
-            if (d == 13 || at || regular) sT[k] = I.buf[nb.par[1][d]][1][gi];
+        // stage phi and T of the block and its 1-cell halo: every load issued
+        // before the first wait (a cell not needed loads from its valid address
+        // anyway and is not stored), from buffer pointers of one scalar batch
+        {
+            constexpr int KA = (A + NT - 1) / NT;
+            const gdouble *bp0 = I.buf[0][0], *bp1 = I.buf[1][0], *bT0 = I.buf[0][1], *bT1 = I.buf[1][1];
+            asm volatile("" : "+s"(bp0), "+s"(bp1), "+s"(bT0), "+s"(bT1));
+            double vp[KA], vT[KA];
+            int dk[KA];
+#pragma unroll
+            for (int q = 0; q < KA; q++) {
+                const int k0 = tid + q * NT, k = k0 < A ? k0 : 0;
+                const int lx = k % WX, ly = (k / WX) % WX, lz = k / (WX * WX);
+                const int d = nb_dir3<TB, TZ, 1>(lx, ly, lz);
+                const int64_t gi = wrapi(x0 + lx - 1, nx) + wrapi(y0 + ly - 1, ny) * (int64_t)nx +
+                                   wrapi(z0 + lz - 1, nz) * nxy;
+                vp[q] = (nb.par[0][d] ? bp1 : bp0)[gi];
+                vT[q] = (nb.par[1][d] ? bT1 : bT0)[gi];
+                dk[q] = d;
+            }
+#pragma unroll
+            for (int q = 0; q < KA; q++) {
+                const int k = tid + q * NT;
+                if (k < A) {
+                    if (dk[q] == 13 || ap) sp[k] = vp[q];
+                    if (dk[q] == 13 || at || regular) sT[k] = vT[q];
+                }
+            }
         }
         __syncthreads();
         const double f0 = 1. / (2 * I.dx), f1 = 1. / I.dx / I.dx, f2 = -I.alpha / 3.14159, f3 = 1. / I.tau;
