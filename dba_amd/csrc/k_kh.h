@@ -153,40 +153,67 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
         const int pself = nb.par[0][4];
 
         // 1. primitives on the tile + 2-cell halo: from U where the owner runs
-        //    this step, else the primitives its ring kept
-        for (int k = tid; k < A2; k += NT) {
-            const int lx = k % W2, ly = k / W2;
-            const int d = nb_dir1<TB, 2>(lx) + 3 * nb_dir1<TB, 2>(ly);
-            const int gy = y0 + ly - 2;
-            const int mx = map_index(x0 + lx - 2, nx, 0), my = map_index(gy, ny, wall ? 1 : 0);
-            const int64_t gi = mx + (int64_t)my * nx;
-            const double sgn = (wall && (gy < 0 || gy >= ny)) ? -1.0 : 1.0;
-            double r, u, v, p;
-            if (nb.act[0][d]) {
-                const int pp = nb.par[0][d];
-                const double um = I.buf[pp][0][gi] - nb.shift[0][d];
-                const double ux = I.buf[pp][1][gi] - nb.shift[1][d];
-                const double uy = I.buf[pp][2][gi] - nb.shift[2][d];
-                const double ue = I.buf[pp][3][gi] - nb.shift[3][d];
-                kh_prims(c, um, ux, uy, ue, &r, &u, &v, &p);
-            } else {
-                // the ring primitives the sleeping owner stored; its inner cells
-                // are never reached by a stencil of a running cell
+        //    this step, else the primitives its ring kept (a sleeping owner's
+        //    inner cells are never reached by a stencil of a running cell). Every
+        //    load of the thread's cells is issued before the first wait: the
+        //    source address is selected per cell (U of the owner's buffer, or its
+        //    ring record), a cell past the region loads cell 0 again, and a cell
+        //    with no ring value loads the record's first word; neither is stored.
+        {
+            constexpr int K1 = (A2 + NT - 1) / NT;
+            const gdouble *u00 = I.buf[0][0], *u01 = I.buf[0][1], *u02 = I.buf[0][2], *u03 = I.buf[0][3];
+            const gdouble *u10 = I.buf[1][0], *u11 = I.buf[1][1], *u12 = I.buf[1][2], *u13 = I.buf[1][3];
+            const gdouble* rec0 = I.aux[0];
+            asm volatile("" : "+s"(u00), "+s"(u01), "+s"(u02), "+s"(u03), "+s"(u10), "+s"(u11), "+s"(u12),
+                         "+s"(u13), "+s"(rec0));
+            double val[K1][4];
+            int dq[K1], riq[K1];
+            double sgq[K1];
+#pragma unroll
+            for (int q = 0; q < K1; q++) {
+                const int k0 = tid + q * NT, k = k0 < A2 ? k0 : 0;
+                const int lx = k % W2, ly = k / W2;
+                const int d = nb_dir1<TB, 2>(lx) + 3 * nb_dir1<TB, 2>(ly);
+                const int gy = y0 + ly - 2;
+                const int mx = map_index(x0 + lx - 2, nx, 0), my = map_index(gy, ny, wall ? 1 : 0);
+                const int64_t gi = mx + (int64_t)my * nx;
                 const int ri = E::ring(mx & (TB - 1), my & (TB - 1));
-                if (ri >= 0) {
-                    const gdouble* e = I.aux[0] + (int64_t)nb.tile[d] * E::R;
-                    r = e[ri];
-                    u = e[E::RING + ri];
-                    v = e[2 * E::RING + ri];
-                    p = e[3 * E::RING + ri];
+                const bool act = nb.act[0][d] != 0;
+                const bool pp = nb.par[0][d] != 0;
+                const gdouble* e = rec0 + (int64_t)nb.tile[d] * E::R + (ri >= 0 ? ri : 0);
+                val[q][0] = act ? (pp ? u10 : u00)[gi] : e[0];
+                val[q][1] = act ? (pp ? u11 : u01)[gi] : e[E::RING];
+                val[q][2] = act ? (pp ? u12 : u02)[gi] : e[2 * E::RING];
+                val[q][3] = act ? (pp ? u13 : u03)[gi] : e[3 * E::RING];
+                dq[q] = d;
+                riq[q] = ri;
+                sgq[q] = (wall && (gy < 0 || gy >= ny)) ? -1.0 : 1.0;
+            }
+#pragma unroll
+            for (int q = 0; q < K1; q++) {
+                const int k = tid + q * NT;
+                if (k >= A2) continue;
+                const int d = dq[q];
+                double r, u, v, p;
+                if (nb.act[0][d]) {
+                    const double um = val[q][0] - nb.shift[0][d];
+                    const double ux = val[q][1] - nb.shift[1][d];
+                    const double uy = val[q][2] - nb.shift[2][d];
+                    const double ue = val[q][3] - nb.shift[3][d];
+                    kh_prims(c, um, ux, uy, ue, &r, &u, &v, &p);
+                } else if (riq[q] >= 0) {
+                    r = val[q][0];
+                    u = val[q][1];
+                    v = val[q][2];
+                    p = val[q][3];
                 } else {
                     r = u = v = p = 0.0;
                 }
+                sP[0 * A2 + k] = r;
+                sP[1 * A2 + k] = u;
+                sP[2 * A2 + k] = sgq[q] * v;
+                sP[3 * A2 + k] = p;
             }
-            sP[0 * A2 + k] = r;
-            sP[1 * A2 + k] = u;
-            sP[2 * A2 + k] = sgn * v;
-            sP[3 * A2 + k] = p;
         }
         __syncthreads();
 
