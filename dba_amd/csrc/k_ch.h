@@ -310,6 +310,12 @@ __device__ __forceinline__ int ch_tile(CInst& I, int t, uint64_t rec, NbTable& n
             smu[(tx + 1) + (ly + 1) * W1] = mu;
         }
     }
+    // the detector's per-wave votes ride on the barrier after the mu ring (no
+    // barriers of its own); sflag is next written after this tile's last barrier
+    if (!regular) {
+        const unsigned long long vote = __ballot(flag);
+        if ((tid & 63) == 0) sflag[tid >> 6] = vote != 0ull;
+    }
     for (int k = tid; k < NH1; k += NT) {
         int lx, ly;
         if (k < W1) { ly = 0; lx = k; }
@@ -330,6 +336,11 @@ __device__ __forceinline__ int ch_tile(CInst& I, int t, uint64_t rec, NbTable& n
         smu[lx + ly * W1] = mu;
     }
     __syncthreads();
+    int any = 0;
+    if (!regular) {
+#pragma unroll
+        for (int w = 0; w < NT / 64; w++) any |= sflag[w];
+    }
 
     // 3. c' on the strip
     double part = 0.0;
@@ -349,7 +360,6 @@ __device__ __forceinline__ int ch_tile(CInst& I, int t, uint64_t rec, NbTable& n
     //    did not fire: a tile that fired is awake next step (the dilation keeps
     //    the tile itself), and a ring is read only while its owner sleeps, so
     //    the ring a sleeping owner presents is the one of the last step it ran
-    const int any = regular ? 0 : block_any<NT>(flag, sflag);
     if (WORKLIST && !regular && tid == 0) I.nflag[0][t] = (uint8_t)any;
     if (!regular && !any) {
         for (int k = tid; k < TileRing<TB>::N; k += NT) {
@@ -526,20 +536,6 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-template <int NT>
-__device__ __forceinline__ int block_any_lds(int pred, int* lds_scratch) {
-    constexpr int NW = NT / 64;
-    const unsigned long long b = __ballot(pred);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) lds_scratch[wid] = b != 0ull;
-    lds_barrier();
-    int r = 0;
-#pragma unroll
-    for (int w = 0; w < NW; w++) r |= lds_scratch[w];
-    lds_barrier();
-    return r;
-}
-
 constexpr int kMarchRing = 64;  // LDS ring rows (a power of two >= 36 + 32 - 4)
 constexpr int kMarchMinTiles = 4096;  // grids from 2048^2 (32 x 32 tiles) march
 
@@ -714,7 +710,16 @@ __global__ __launch_bounds__(256) void ch_march_kernel(const Inst* __restrict__ 
                 }
                 smu[lx + ly * W1] = mu;
             }
+            if (!regular) {  // detector votes ride on this barrier (see ch_tile)
+                const unsigned long long vote = __ballot(flag);
+                if ((tid & 63) == 0) sflag[tid >> 6] = vote != 0ull;
+            }
             lds_barrier();
+            int anyv = 0;
+            if (!regular) {
+#pragma unroll
+                for (int w = 0; w < NT / 64; w++) anyv |= sflag[w];
+            }
             // c' on the strip
             {
                 double Lm[R + 2], Cm[R + 2], Rm[R + 2];
@@ -739,7 +744,7 @@ __global__ __launch_bounds__(256) void ch_march_kernel(const Inst* __restrict__ 
                     nxt[x0 + tx + (int64_t)((ty0 + j) * TB + ty * R + r) * nx] = c[r] + change;
                 }
             }
-            any = regular ? 0 : block_any_lds<NT>(flag, sflag);
+            any = anyv;
             if (!regular && !any) {
                 for (int k = tid; k < TileRing<TB>::N; k += NT) {
                     int lx, ly;
