@@ -379,7 +379,7 @@ __device__ __forceinline__ int ch_tile(CInst& I, int t, uint64_t rec, NbTable& n
             }
         }
     }
-    __syncthreads();
+    if (WORKLIST) __syncthreads();  // LDS reuse by the next item; a single-launch tile is the last
     return any;
 }
 
@@ -745,6 +745,10 @@ __global__ __launch_bounds__(256) void ch_march_kernel(const Inst* __restrict__ 
                 }
             }
             any = anyv;
+            // tile j read its window (ring rows 32 j .. 32 j + 35) only before the
+            // mu barrier, so the next tile's rows, whose last 4 wrap onto this
+            // window's first 4, can be written now
+            if (pre) deposit(32 * (j + 1) + 4);
             if (!regular && !any) {
                 for (int k = tid; k < TileRing<TB>::N; k += NT) {
                     int lx, ly;
@@ -752,6 +756,8 @@ __global__ __launch_bounds__(256) void ch_march_kernel(const Inst* __restrict__ 
                     mu_store[(int64_t)t * TileRing<TB>::N + k] = smu[(lx + 1) + (ly + 1) * W1];
                 }
             }
+        } else if (pre) {
+            deposit(32 * (j + 1) + 4);  // a sleeping tile reads no ring rows
         }
         if (tid == 0) {
             I.dF[next][t] = (uint8_t)any;
@@ -764,8 +770,9 @@ __global__ __launch_bounds__(256) void ch_march_kernel(const Inst* __restrict__ 
                 }
             }
         }
-        lds_barrier();  // every read of tile j's window and mu is done
-        if (pre) deposit(32 * (j + 1) + 4);
+        // the next tile's rows were deposited after the mu barrier (below the
+        // c' pass) or at once by a sleeping tile; this barrier makes them
+        // visible and ends every read of mu
         lds_barrier();
     }
 }
