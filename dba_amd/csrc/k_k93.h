@@ -97,8 +97,45 @@ __device__ __forceinline__ int k93_tile(CInst& I, int t, uint64_t rec, NbTable& 
     gdouble* __restrict__ Tnew = pp1 ? I.buf[0][1] : I.buf[1][1];
 
     // stage phi (2-cell halo) when p-awake, T (1-cell halo) when T-awake,
-    // and phi / T of the tile itself always (the T update needs phi' - phi)
-    if (!STAGED)
+    // and phi / T of the tile itself always (the T update needs phi' - phi).
+    // A full tile issues every load of the thread's cells before the first wait
+    // (a cell not needed, or past the region, loads a valid cell and is not
+    // stored); a partial tile takes the generic loops below.
+    if (!STAGED && !PARTIAL) {
+        constexpr int KP = (W2 * W2 + NT - 1) / NT, KT = (W1 * W1 + NT - 1) / NT;
+        double vp[KP], vT[KT];
+#pragma unroll
+        for (int q = 0; q < KP; q++) {
+            const int k0 = tid + q * NT, k = k0 < W2 * W2 ? k0 : 0;
+            const int lx = k % W2, ly = k / W2;
+            const int mx = map_index(x0 + lx - 2, nx, bcx), my = map_index(y0 + ly - 2, ny, bcy);
+            const int d = nb_dir1<TB, 2>(lx) + 3 * nb_dir1<TB, 2>(ly);
+            vp[q] = (nb.par[0][d] ? p1 : p0)[mx + (int64_t)my * nx];
+        }
+#pragma unroll
+        for (int q = 0; q < KT; q++) {
+            const int k0 = tid + q * NT, k = k0 < W1 * W1 ? k0 : 0;
+            const int lx = k % W1, ly = k / W1;
+            const int mx = map_index(x0 + lx - 1, nx, bcx), my = map_index(y0 + ly - 1, ny, bcy);
+            const int d = nb_dir1<TB, 1>(lx) + 3 * nb_dir1<TB, 1>(ly);
+            vT[q] = (nb.par[1][d] ? T1 : T0)[mx + (int64_t)my * nx];
+        }
+#pragma unroll
+        for (int q = 0; q < KP; q++) {
+            const int k = tid + q * NT;
+            const int lx = k % W2, ly = k / W2;
+            const bool own = lx >= 2 && lx < TB + 2 && ly >= 2 && ly < TB + 2;
+            if (k < W2 * W2 && (own || ap)) sp[k] = vp[q];
+        }
+#pragma unroll
+        for (int q = 0; q < KT; q++) {
+            const int k = tid + q * NT;
+            const int lx = k % W1, ly = k / W1;
+            const bool own = lx >= 1 && lx <= TB && ly >= 1 && ly <= TB;
+            if (k < W1 * W1 && (own || at)) sT[k] = vT[q];
+        }
+    }
+    if (!STAGED && PARTIAL)
     for (int k = tid; k < (w + 4) * (h + 4); k += NT) {
         const int lx = k % (w + 4), ly = k / (w + 4);
         const bool own = lx >= 2 && lx < w + 2 && ly >= 2 && ly < h + 2;
@@ -107,7 +144,7 @@ __device__ __forceinline__ int k93_tile(CInst& I, int t, uint64_t rec, NbTable& 
         const int d = PARTIAL ? (own ? 4 : odir(mx, my)) : nb_dir1<TB, 2>(lx) + 3 * nb_dir1<TB, 2>(ly);
         sp[lx + ly * W2] = (nb.par[0][d] ? p1 : p0)[mx + (int64_t)my * nx];
     }
-    if (!STAGED)
+    if (!STAGED && PARTIAL)
     for (int k = tid; k < (w + 2) * (h + 2); k += NT) {
         const int lx = k % (w + 2), ly = k / (w + 2);
         const bool own = lx >= 1 && lx <= w && ly >= 1 && ly <= h;
@@ -186,7 +223,9 @@ __device__ __forceinline__ bool k93_partial(CInst& I, int t) {
 }
 
 template <int TB, int NT, bool PART>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k93_step_kernel(const WorkList wl) {
+// 4 waves per SIMD: the staging's loads in flight (7 doubles per thread at 128
+// threads) fit without spills (5 waves spilled 48 B per lane and ran slower)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k93_step_kernel(const WorkList wl) {
     __shared__ K93Lds<TB> L;
     __shared__ int sflag[NT / 64 + 1];
     __shared__ NbTable nb;
