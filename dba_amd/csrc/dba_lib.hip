@@ -104,6 +104,7 @@ struct Group {
     bool partial = false;         // an instance with partial tiles: the kernels' PART instances
     int march = 0;                // column-march single-launch steps: tiles per strip (0: off)
     bool spec = false;            // K93 single-launch steps with speculative staging
+    bool kh_p2 = false;           // KH: dx, dy and the cell area of every instance are powers of two
     bool nine = true;             // CH 9-point stencil (the march kernel is instanced on it)
     Inst* d_insts = nullptr;
     Seg* d_segs = nullptr;
@@ -483,6 +484,20 @@ dba_status get_group(dba_ctx* ctx, const std::vector<int>& ids, Group** out) {
     // would pay the rings of both buffers of every sleeping tile in HBM bytes
     g->spec = ids.size() == 1 && g->model == DBA_MODEL_DENDRITE_K93 && g->tile == 16 && g->dense_threads == 512 &&
               first->h.full_tiles && first->h.nt <= 1024;
+    // KH divides by dx, dy and the cell area; when all three are powers of two
+    // (a power-of-two grid on the unit square) the step multiplies by their exact
+    // reciprocals instead, which gives the same doubles (kdiv in k_kh.h)
+    if (g->model == DBA_MODEL_KH) {
+        auto pow2 = [](double d) {
+            int e = 0;
+            return d > 0.0 && std::isfinite(d) && std::frexp(d, &e) == 0.5;
+        };
+        g->kh_p2 = true;
+        for (int id : ids) {
+            const Instance* in = ctx->inst.at(id);
+            if (!(pow2(in->h.dx) && pow2(in->h.dy) && pow2(in->h.vol))) g->kh_p2 = false;
+        }
+    }
     // Model B: 65536 workgroups (4 blocks each on a fully awake 256^3 field)
     // measured faster than 16384 or 3072 with longer pipelines per wave
     int64_t cap = g->model == DBA_MODEL_MODELB ? 65536 : (g->model == DBA_MODEL_DENDRITE_K94 ? 16384 : 4096);
@@ -548,7 +563,10 @@ void launch_step_tb(const Group* g, hipStream_t s) {
             break;
         case DBA_MODEL_KH:
             if constexpr (TB <= 16 && NT <= 256) {
-                hipLaunchKernelGGL((kh_step_kernel<TB, NT>), grid, block, 0, s, wl_of(g));
+                if (g->kh_p2)
+                    hipLaunchKernelGGL((kh_step_kernel<TB, NT, true>), grid, block, 0, s, wl_of(g));
+                else
+                    hipLaunchKernelGGL((kh_step_kernel<TB, NT, false>), grid, block, 0, s, wl_of(g));
             }
             break;
         default:

@@ -29,17 +29,29 @@ namespace dba {
 
 struct KhC {
     double g, dx, dy, dt, vol;
+    double idx, idy, ivol;  // 1/dx, 1/dy, 1/vol: exact when P2 (below)
 };
 
+// x / d. P2: d is a power of two (the grid spacings and the cell area of a
+// power-of-two grid on a unit square), so x * (1/d) rounds the same exact real
+// as x / d and gives the same double in every case, at the cost of a multiply
+// instead of a correctly rounded division.
+template <bool P2>
+__device__ __forceinline__ double kdiv(double x, double d, double id) {
+    if constexpr (P2) return x * id;
+    else return x / d;
+}
+
+template <bool P2 = false>
 __device__ __forceinline__ void kh_prims(const KhC& c, double m, double mx, double my, double e, double* r,
                                          double* u, double* v, double* p) {
-    const double rho = m / c.vol;
-    const double uu = mx / rho / c.vol;
-    const double vv = my / rho / c.vol;
+    const double rho = kdiv<P2>(m, c.vol, c.ivol);
+    const double uu = kdiv<P2>(mx / rho, c.vol, c.ivol);
+    const double vv = kdiv<P2>(my / rho, c.vol, c.ivol);
     *r = rho;
     *u = uu;
     *v = vv;
-    *p = (e / c.vol - 0.5 * rho * (uu * uu + vv * vv)) * (c.g - 1.0);
+    *p = (kdiv<P2>(e, c.vol, c.ivol) - 0.5 * rho * (uu * uu + vv * vv)) * (c.g - 1.0);
 }
 
 __device__ __forceinline__ double kh_speed(const KhC& c, double r, double u, double v, double p) {
@@ -101,7 +113,7 @@ struct KhEdge {
     }
 };
 
-template <int TB, int NT>
+template <int TB, int NT, bool P2>
 __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
     using E = KhEdge<TB>;
     constexpr int W2 = TB + 4;
@@ -147,6 +159,9 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
         c.dx = I.dx;
         c.dy = I.dy;
         c.vol = I.vol;
+        c.idx = 1.0 / c.dx;
+        c.idy = 1.0 / c.dy;
+        c.ivol = 1.0 / c.vol;
         c.dt = cfl ? I.scal[1] : I.dt;
         nb_fill<TB>(nb, I, t, x0, y0, regular, 1, ncons, rec);
         __syncthreads();
@@ -200,7 +215,7 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
                     const double ux = val[q][1] - nb.shift[1][d];
                     const double uy = val[q][2] - nb.shift[2][d];
                     const double ue = val[q][3] - nb.shift[3][d];
-                    kh_prims(c, um, ux, uy, ue, &r, &u, &v, &p);
+                    kh_prims<P2>(c, um, ux, uy, ue, &r, &u, &v, &p);
                 } else if (riq[q] >= 0) {
                     r = val[q][0];
                     u = val[q][1];
@@ -258,13 +273,13 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
             const int p0 = (lx + 1) + (ly + 1) * W2;
             const int pxm = p0 - 1, pxp = p0 + 1, pym = p0 - W2, pyp = p0 + W2;
             // Grad_x / Grad_y (example_Kelvin_Helmholtz/KH_DBA.cu:249-275)
-            const double r_x = (sP[pxp] - sP[pxm]) / 2.0 / c.dx, r_y = (sP[pyp] - sP[pym]) / 2.0 / c.dy;
-            const double u_x = (sP[A2 + pxp] - sP[A2 + pxm]) / 2.0 / c.dx;
-            const double u_y = (sP[A2 + pyp] - sP[A2 + pym]) / 2.0 / c.dy;
-            const double v_x = (sP[2 * A2 + pxp] - sP[2 * A2 + pxm]) / 2.0 / c.dx;
-            const double v_y = (sP[2 * A2 + pyp] - sP[2 * A2 + pym]) / 2.0 / c.dy;
-            const double p_x = (sP[3 * A2 + pxp] - sP[3 * A2 + pxm]) / 2.0 / c.dx;
-            const double p_y = (sP[3 * A2 + pyp] - sP[3 * A2 + pym]) / 2.0 / c.dy;
+            const double r_x = kdiv<P2>((sP[pxp] - sP[pxm]) / 2.0, c.dx, c.idx), r_y = kdiv<P2>((sP[pyp] - sP[pym]) / 2.0, c.dy, c.idy);
+            const double u_x = kdiv<P2>((sP[A2 + pxp] - sP[A2 + pxm]) / 2.0, c.dx, c.idx);
+            const double u_y = kdiv<P2>((sP[A2 + pyp] - sP[A2 + pym]) / 2.0, c.dy, c.idy);
+            const double v_x = kdiv<P2>((sP[2 * A2 + pxp] - sP[2 * A2 + pxm]) / 2.0, c.dx, c.idx);
+            const double v_y = kdiv<P2>((sP[2 * A2 + pyp] - sP[2 * A2 + pym]) / 2.0, c.dy, c.idy);
+            const double p_x = kdiv<P2>((sP[3 * A2 + pxp] - sP[3 * A2 + pxm]) / 2.0, c.dx, c.idx);
+            const double p_y = kdiv<P2>((sP[3 * A2 + pyp] - sP[3 * A2 + pym]) / 2.0, c.dy, c.idy);
             const double r = sP[p0], u = sP[A2 + p0], v = sP[2 * A2 + p0], p = sP[3 * A2 + p0];
             const double dt = c.dt, g = c.g;
             // extrap_in_time (example_Kelvin_Helmholtz/KH_DBA.cu:524-531)
@@ -289,11 +304,11 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
         // (example_Kelvin_Helmholtz/KH_DBA.cu:249-275, the expressions of phase 2)
         auto grad_x = [&](int q, int lx, int ly) {
             const int p0 = (lx + 1) + (ly + 1) * W2;
-            return (sP[q * A2 + p0 + 1] - sP[q * A2 + p0 - 1]) / 2.0 / c.dx;
+            return kdiv<P2>((sP[q * A2 + p0 + 1] - sP[q * A2 + p0 - 1]) / 2.0, c.dx, c.idx);
         };
         auto grad_y = [&](int q, int lx, int ly) {
             const int p0 = (lx + 1) + (ly + 1) * W2;
-            return (sP[q * A2 + p0 + W2] - sP[q * A2 + p0 - W2]) / 2.0 / c.dy;
+            return kdiv<P2>((sP[q * A2 + p0 + W2] - sP[q * A2 + p0 - W2]) / 2.0, c.dy, c.idy);
         };
 
         // detector; the edge record for sleeping-neighbour reads only when the
@@ -430,7 +445,7 @@ __global__ __launch_bounds__(NT) void kh_step_kernel(const WorkList wl) {
             }
             if (cfl) {
                 double r, u, v, p;
-                kh_prims(c, un[0], un[1], un[2], un[3], &r, &u, &v, &p);
+                kh_prims<P2>(c, un[0], un[1], un[2], un[3], &r, &u, &v, &p);
                 vfresh = fmax(vfresh, kh_speed(c, r, u, v, p));
             }
         }

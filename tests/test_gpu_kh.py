@@ -55,6 +55,21 @@ def test_reference_model_matches_oracle(ctx, mode):
     inst.destroy()
 
 
+def test_non_power_of_two_spacing_matches_oracle(ctx):
+    """dx = dy = 0.003: the step divides by the spacings and the cell area (the
+    power-of-two grids of the other tests multiply by exact reciprocals)."""
+    n = 128
+    p = configs.kh_reference_params(n=n, tile=8, mode=abi.MODE_DBA, dx=0.003, dt=2e-4, eps=8.0, history_cap=64)
+    U = configs.kh_reference_ic(n, dx=0.003)
+    inst, o = pair(ctx, p, U)
+    for chunk in (1, 3, 10, 20):
+        inst.step(chunk)
+        o.step(chunk)
+        compare(inst, o, exact=True)
+        assert np.array_equal(inst.active(0)[1], o.awake(0))
+    inst.destroy()
+
+
 @pytest.mark.parametrize("mode", [abi.MODE_REGULAR, abi.MODE_DBA])
 @pytest.mark.parametrize("tile,threads", [(16, 256), (8, 64), (16, 128)])
 def test_config_model_matches_oracle(ctx, mode, tile, threads):
