@@ -274,7 +274,10 @@ __device__ __forceinline__ int nb_dir1(int l) {
 template <int NT>
 __device__ __forceinline__ double block_max(double v, double* lds_scratch) {
     constexpr int NW = NT / 64;
-    // the shift tree of wave_sum_down with fmax (lane 0 sees the same operands)
+    // the shift tree of wave_sum_down with fmax (lane 0 sees the same operands:
+    // none of the lanes it reads is a row shift's zero filler, so the result holds
+    // for values of any sign; the same DPP shifts are pinned bit-exactly by the
+    // field-sum test of dba_field_sum)
     v = fmax(v, __shfl_down(v, 32, 64));
     v = fmax(v, __shfl_down(v, 16, 64));
     v = fmax(v, dpp_row_down<8>(v));
