@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -105,6 +106,8 @@ struct Group {
     int march = 0;                // column-march single-launch steps: tiles per strip (0: off)
     bool spec = false;            // K93 single-launch steps with speculative staging
     bool kh_p2 = false;           // KH: dx, dy and the cell area of every instance are powers of two
+    int sb_from = 0;              // Model B: lists of at least this many blocks step on superblocks (0: never)
+    int sb_z = 1;                 // ... of 2 x 2 x sb_z blocks
     bool nine = true;             // CH 9-point stencil (the march kernel is instanced on it)
     Inst* d_insts = nullptr;
     Seg* d_segs = nullptr;
@@ -502,6 +505,22 @@ dba_status get_group(dba_ctx* ctx, const std::vector<int>& ids, Group** out) {
     // measured faster than 16384 or 3072 with longer pipelines per wave
     int64_t cap = g->model == DBA_MODEL_MODELB ? 65536 : (g->model == DBA_MODEL_DENDRITE_K94 ? 16384 : 4096);
     g->grid = (int)std::max<int64_t>(1, std::min<int64_t>(g->total_tiles, cap));
+    // Model B superblocks (modelb_mu_sb_kernel): one 3D instance of full 4x4x4
+    // blocks, an even number of blocks per axis, no conserved shift; steps with
+    // at least a quarter of the blocks awake. DBA_MODELB_SUPERBLOCKS=0 turns them
+    // off and =1 uses them for every non-empty list (both give the same doubles).
+    if (g->model == DBA_MODEL_MODELB && ids.size() == 1 && g->tile == 4 && g->tile_z == 4) {
+        const Inst& h = first->h;
+        if (first->p.mode != DBA_MODE_CONSERVED && h.full_tiles && h.nz > 1 && h.ntx % 2 == 0 &&
+            h.nty % 2 == 0 && h.ntz % g->sb_z == 0) {
+            const char* env = std::getenv("DBA_MODELB_SUPERBLOCKS");
+            const std::string e = env ? env : "";
+            g->sb_from = e == "0" ? 0 : (e == "1" ? 1 : (int)((h.nt + 3) / 4));
+            // the block kernels then only take lists shorter than sb_from
+            if (g->sb_from > 0)
+                g->grid = (int)std::max<int64_t>(1, std::min<int64_t>(g->grid, (g->sb_from + 7) / 8));
+        }
+    }
     size_t n = ids.size();
     HIP_TRY(hipMalloc(&g->d_insts, sizeof(Inst) * n));
     HIP_TRY(hipMemcpyAsync(g->d_insts, hs.data(), sizeof(Inst) * n, hipMemcpyHostToDevice, ctx->setup));
@@ -589,12 +608,27 @@ void launch_step(const Group* g, hipStream_t s) {
     if (g->model == DBA_MODEL_DENDRITE_K94 || g->model == DBA_MODEL_MODELB) {
         dim3 grid(g->grid), block(64);
         if (g->model == DBA_MODEL_MODELB) {
+            // each pair returns at once when the other takes the step (by list length)
+            const int from = g->sb_from > 0 ? g->sb_from : INT_MAX;
+            const dim3 sgrid((unsigned)std::max<int64_t>(1, g->total_tiles / (4 * g->sb_z))), sblock(256 * g->sb_z);
             if (g->tile_z == 4) {
-                hipLaunchKernelGGL((modelb_mu_kernel<4, 4>), grid, block, 0, s, wl_of(g));
-                hipLaunchKernelGGL((modelb_update_kernel<4, 4>), grid, block, 0, s, wl_of(g));
+                if (g->sb_from > 0) {
+                    if (g->sb_z == 2)
+                        hipLaunchKernelGGL(modelb_mu_sb_kernel<2>, sgrid, sblock, 0, s, wl_of(g), from);
+                    else
+                        hipLaunchKernelGGL(modelb_mu_sb_kernel<1>, sgrid, sblock, 0, s, wl_of(g), from);
+                }
+                hipLaunchKernelGGL((modelb_mu_kernel<4, 4>), grid, block, 0, s, wl_of(g), from);
+                if (g->sb_from > 0) {
+                    if (g->sb_z == 2)
+                        hipLaunchKernelGGL(modelb_update_sb_kernel<2>, sgrid, sblock, 0, s, wl_of(g), from);
+                    else
+                        hipLaunchKernelGGL(modelb_update_sb_kernel<1>, sgrid, sblock, 0, s, wl_of(g), from);
+                }
+                hipLaunchKernelGGL((modelb_update_kernel<4, 4>), grid, block, 0, s, wl_of(g), from);
             } else {
-                hipLaunchKernelGGL((modelb_mu_kernel<4, 1>), grid, block, 0, s, wl_of(g));
-                hipLaunchKernelGGL((modelb_update_kernel<4, 1>), grid, block, 0, s, wl_of(g));
+                hipLaunchKernelGGL((modelb_mu_kernel<4, 1>), grid, block, 0, s, wl_of(g), from);
+                hipLaunchKernelGGL((modelb_update_kernel<4, 1>), grid, block, 0, s, wl_of(g), from);
             }
         } else {
             if (g->tile_z == 4)

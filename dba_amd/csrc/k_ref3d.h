@@ -254,9 +254,54 @@ __device__ __forceinline__ int64_t mb_own_index(const MbBlock& b, int lane) {
     return (b.x0 + lx) + (int64_t)(b.y0 + ly) * I.nx + (int64_t)(b.z0 + lz) * ((int64_t)I.nx * I.ny);
 }
 
+// The chemical potentials of the cell at (cx, cy, cz) of an LDS box of x, y, z
+// (field f at sc + f AP) into df; returns lap z for the detector
+// (example_Reactive_ModelB/kernel.cu:557-587).
+template <int RX, int PZ, int AP>
+__device__ __forceinline__ double mb_mu_at(CInst& I, const double* sc, int cx, int cy, int cz, double phi,
+                                           double (&df)[3]) {
+    const int q = cx + cy * RX + cz * PZ;
+    const double X = sc[q], Y = sc[AP + q], Z = sc[2 * AP + q];
+    const double B = 1.0 - X - Y - Z;
+    // logarithms first, then one Laplacian at a time: left free, the
+    // scheduler hoists all 81 LDS reads above the logarithms (250 VGPRs)
+    const double logB = log_f64(B), logX = log_f64(X), logY = log_f64(Y), logZ = log_f64(Z);
+    __builtin_amdgcn_sched_barrier(0);
+    const double inv_dx2 = I.invdx2;
+    const double lx_ = lds_lap27<RX, PZ>(sc, cx, cy, cz, inv_dx2);
+    __builtin_amdgcn_sched_barrier(0);
+    const double ly_ = lds_lap27<RX, PZ>(sc + AP, cx, cy, cz, inv_dx2);
+    __builtin_amdgcn_sched_barrier(0);
+    const double lz_ = lds_lap27<RX, PZ>(sc + 2 * AP, cx, cy, cz, inv_dx2);
+    df[0] = I.mb_cx + I.chi_xy * Y + I.chi_xz * Z + I.chi_xb * B - I.chi_xb * X - I.chi_yb * Y - I.chi_zb * Z +
+            logX / I.r_x - logB - I.epsx_sq * lx_;
+    df[1] = I.mb_cy + I.chi_xy * X + I.chi_yz * Z + I.chi_yb * B - I.chi_xb * X - I.chi_yb * Y - I.chi_zb * Z +
+            logY / I.r_y - logB - I.epsy_sq * ly_;
+    df[2] = I.mb_cz + I.chi_xz * X + I.chi_yz * Y + I.chi_zb * B - I.chi_xb * X - I.chi_yb * Y - I.chi_zb * Z +
+            logZ / I.r_z - logB - I.p_gel * phi * phi / 2.0 / I.mb_omz - I.epsz_sq * lz_ - I.mb_lk;
+    return lz_;
+}
+
+// The changes of x, y, z at (cx, cy, cz) from an LDS box of the potentials
+// (example_Reactive_ModelB/kernel.cu:419-432): dt v_f (mob_f lap mu_f -/+ the reaction)
+template <int RX, int PZ, int AP>
+__device__ __forceinline__ void mb_change_at(CInst& I, const double* smu, int cx, int cy, int cz,
+                                             double (&ch)[3]) {
+    const double inv_dx2 = I.invdx2;
+    const int mq = cx + cy * RX + cz * PZ;
+    const double dfx = smu[mq], dfy = smu[AP + mq], dfz = smu[2 * AP + mq];
+    const double R = I.k_0 * (exp(I.n_exp * I.v_x * dfx + I.m_exp * I.v_y * dfy) - exp(I.v_z * dfz));
+    __builtin_amdgcn_sched_barrier(0);
+    ch[0] = I.dt * I.v_x * (I.mob_x * lds_lap27<RX, PZ>(smu, cx, cy, cz, inv_dx2) - I.n_exp * R);
+    __builtin_amdgcn_sched_barrier(0);
+    ch[1] = I.dt * I.v_y * (I.mob_y * lds_lap27<RX, PZ>(smu + AP, cx, cy, cz, inv_dx2) - I.m_exp * R);
+    __builtin_amdgcn_sched_barrier(0);
+    ch[2] = I.dt * I.v_z * (I.mob_z * lds_lap27<RX, PZ>(smu + 2 * AP, cx, cy, cz, inv_dx2) + R);
+}
+
 // Calc_mu(_wakeup) + the detector |lap z| > eps (example_Reactive_ModelB/kernel.cu:290-420)
 template <int TB, int TZ>
-__global__ __launch_bounds__(64) void modelb_mu_kernel(const WorkList wl) {
+__global__ __launch_bounds__(64) void modelb_mu_kernel(const WorkList wl, int sb_from) {
     using B1 = Box1<TB, TZ>;
     constexpr int A = B1::A, NH = B1::NH, C = TB * TB * TZ, RX = B1::RX, PZ = B1::PZ, AP = B1::AP;
     static_assert(C <= 64, "at most one cell per lane");
@@ -301,6 +346,8 @@ __global__ __launch_bounds__(64) void modelb_mu_kernel(const WorkList wl) {
     };
 
     const int total = wl.offsets[wl.nseg];
+    // a list of at least sb_from blocks is the superblock kernels' step
+    if (total >= sb_from) return;
     const BandSweep bs = band_sweep_setup(wl, total);
     for (int round0 = 0; (int64_t)round0 * gridDim.x < total; round0 += 64) {
         const RoundList L = bs.cx ? band_sweep_round(bs, round0) : round_list(wl, total, round0);
@@ -353,31 +400,12 @@ __global__ __launch_bounds__(64) void modelb_mu_kernel(const WorkList wl) {
             int flag = 0;
             if (lane < C) {
                 const int lx = lane % TB, ly = (lane / TB) % TB, lz = lane / (TB * TB);
-                const int cx = lx + 1, cy = ly + 1, cz = lz + 1;
-                const int q = cx + cy * RX + cz * PZ;
-                const double X = sc[q], Y = sc[AP + q], Z = sc[2 * AP + q];
-                const double B = 1.0 - X - Y - Z;
-                // logarithms first, then one Laplacian at a time: left free, the
-                // scheduler hoists all 81 LDS reads above the logarithms (250 VGPRs)
-                const double logB = log_f64(B), logX = log_f64(X), logY = log_f64(Y), logZ = log_f64(Z);
-                __builtin_amdgcn_sched_barrier(0);
-                const double inv_dx2 = I.invdx2;
-                const double lx_ = lds_lap27<RX, PZ>(sc, cx, cy, cz, inv_dx2);
-                __builtin_amdgcn_sched_barrier(0);
-                const double ly_ = lds_lap27<RX, PZ>(sc + AP, cx, cy, cz, inv_dx2);
-                __builtin_amdgcn_sched_barrier(0);
-                const double lz_ = lds_lap27<RX, PZ>(sc + 2 * AP, cx, cy, cz, inv_dx2);
-                const double dfx = I.mb_cx + I.chi_xy * Y + I.chi_xz * Z + I.chi_xb * B - I.chi_xb * X -
-                                   I.chi_yb * Y - I.chi_zb * Z + logX / I.r_x - logB - I.epsx_sq * lx_;
-                const double dfy = I.mb_cy + I.chi_xy * X + I.chi_yz * Z + I.chi_yb * B - I.chi_xb * X -
-                                   I.chi_yb * Y - I.chi_zb * Z + logY / I.r_y - logB - I.epsy_sq * ly_;
-                const double dfz = I.mb_cz + I.chi_xz * X + I.chi_yz * Y + I.chi_zb * B - I.chi_xb * X -
-                                   I.chi_yb * Y - I.chi_zb * Z + logZ / I.r_z - logB -
-                                   I.p_gel * phi * phi / 2.0 / I.mb_omz - I.epsz_sq * lz_ - I.mb_lk;
+                double df[3];
+                const double lz_ = mb_mu_at<RX, PZ, AP>(I, sc, lx + 1, ly + 1, lz + 1, phi, df);
                 const int64_t gi = mb_own_index<TB, TZ>(b, lane);
-                I.aux[0][gi] = dfx;
-                I.aux[1][gi] = dfy;
-                I.aux[2][gi] = dfz;
+                I.aux[0][gi] = df[0];
+                I.aux[1][gi] = df[1];
+                I.aux[2][gi] = df[2];
                 flag = fabs(lz_) > I.thresh_a;
             }
             if (I.mode != 0) {
@@ -399,7 +427,7 @@ __global__ __launch_bounds__(64) void modelb_mu_kernel(const WorkList wl) {
 // Update(_wakeup) + copy-back + conserved change capture
 // (example_Reactive_ModelB/kernel.cu:430-560, Conserved/kernel.cu:630-702)
 template <int TB, int TZ>
-__global__ __launch_bounds__(64) void modelb_update_kernel(const WorkList wl) {
+__global__ __launch_bounds__(64) void modelb_update_kernel(const WorkList wl, int sb_from) {
     using B1 = Box1<TB, TZ>;
     constexpr int A = B1::A, NH = B1::NH, C = TB * TB * TZ, RX = B1::RX, PZ = B1::PZ, AP = B1::AP;
     static_assert(C <= 64, "at most one cell per lane");
@@ -443,6 +471,8 @@ __global__ __launch_bounds__(64) void modelb_update_kernel(const WorkList wl) {
     };
 
     const int total = wl.offsets[wl.nseg];
+    // a list of at least sb_from blocks is the superblock kernels' step
+    if (total >= sb_from) return;
     const BandSweep bs = band_sweep_setup(wl, total);
     for (int round0 = 0; (int64_t)round0 * gridDim.x < total; round0 += 64) {
         const RoundList L = bs.cx ? band_sweep_round(bs, round0) : round_list(wl, total, round0);
@@ -489,26 +519,15 @@ __global__ __launch_bounds__(64) void modelb_update_kernel(const WorkList wl) {
                 const int lx = lane % TB, ly = (lane / TB) % TB, lz = lane / (TB * TB);
                 double x[3] = {v[0], v[1], v[2]};
                 for (int f = 0; f < ncons; f++) x[f] -= I.scal[8 + f] - e[f];
-                const double inv_dx2 = I.invdx2;
-                const int mq = (lx + 1) + (ly + 1) * RX + (lz + 1) * PZ;
-                const double dfx = smu[mq], dfy = smu[AP + mq], dfz = smu[2 * AP + mq];
-                const double R = I.k_0 * (exp(I.n_exp * I.v_x * dfx + I.m_exp * I.v_y * dfy) - exp(I.v_z * dfz));
-                __builtin_amdgcn_sched_barrier(0);
-                const double chx =
-                    I.dt * I.v_x * (I.mob_x * lds_lap27<RX, PZ>(smu, lx + 1, ly + 1, lz + 1, inv_dx2) - I.n_exp * R);
-                __builtin_amdgcn_sched_barrier(0);
-                const double chy = I.dt * I.v_y *
-                                   (I.mob_y * lds_lap27<RX, PZ>(smu + AP, lx + 1, ly + 1, lz + 1, inv_dx2) - I.m_exp * R);
-                __builtin_amdgcn_sched_barrier(0);
-                const double chz =
-                    I.dt * I.v_z * (I.mob_z * lds_lap27<RX, PZ>(smu + 2 * AP, lx + 1, ly + 1, lz + 1, inv_dx2) + R);
+                double ch[3];
+                mb_change_at<RX, PZ, AP>(I, smu, lx + 1, ly + 1, lz + 1, ch);
                 const int64_t gi = mb_own_index<TB, TZ>(b, lane);
-                I.buf[pself ^ 1][0][gi] = x[0] + chx;
-                I.buf[pself ^ 1][1][gi] = x[1] + chy;
-                I.buf[pself ^ 1][2][gi] = x[2] + chz;
-                part[0] = chx;
-                part[1] = chy;
-                part[2] = chz;
+                I.buf[pself ^ 1][0][gi] = x[0] + ch[0];
+                I.buf[pself ^ 1][1][gi] = x[1] + ch[1];
+                I.buf[pself ^ 1][2][gi] = x[2] + ch[2];
+                part[0] = ch[0];
+                part[1] = ch[1];
+                part[2] = ch[2];
             }
             for (int f = 0; f < ncons; f++) {
                 const double tot = wave_sum_down(part[f]);
@@ -529,6 +548,248 @@ __global__ __launch_bounds__(64) void modelb_update_kernel(const WorkList wl) {
             w1 = w2;
             has1 = has2;
         }
+    }
+}
+
+// ------------------------------------------------------------ Model B, superblocks
+// Steps of one 3D Model B instance (4x4x4 blocks, no conserved shift) in which
+// at least sb_from blocks are awake run on superblocks of 2 x 2 x SZ blocks: one
+// workgroup of 4 SZ waves stages the box of a superblock's 8 x 8 x 4SZ cells and
+// their 1-cell halo once (10 x 10 x (4SZ + 2) values per field; each block's own
+// box holds 216 values for 64 cells), and wave w computes block w of the
+// superblock if that block is awake. The arithmetic per cell is mb_mu_at /
+// mb_change_at, as in the block kernels, so both give the same doubles. Each
+// superblock kernel returns at once when the list is shorter than sb_from, and
+// the block kernels when it is not.
+
+// positions of the blocks around a superblock among the 4 x 4 x 4 blocks its box
+// can touch: bit (ox + 1) + 4 (oy + 1) + 16 (oz + 1), block offsets o in [-1, 2]
+__device__ __forceinline__ int sb_code(int w) { return 21 + (w & 1) + 4 * ((w >> 1) & 1) + 16 * (w >> 2); }
+
+template <int SZ>
+struct Sb {
+    static constexpr int W = 10, WZ = 4 * SZ + 2, A = W * W * WZ;  // box sides and size
+    static constexpr int NT = 256 * SZ;                            // 4 SZ waves
+    static constexpr int NE = (A + NT - 1) / NT;                   // box elements per thread
+    static constexpr uint64_t own() {
+        uint64_t m = 0;
+        for (int w = 0; w < 4 * SZ; w++) m |= 1ull << (21 + (w & 1) + 4 * ((w >> 1) & 1) + 16 * (w >> 2));
+        return m;
+    }
+};
+
+// A superblock as every wave of its workgroup sees it (all uniform).
+struct MbSuper {
+    int X0, Y0, Z0;  // first cell
+    int tw;          // this wave's block
+    uint64_t pm;     // parity of the blocks the box touches (bit 1: buf[1] is current)
+    uint64_t am;     // awake bits of the own blocks, at the same positions
+    bool interior;   // the box needs no periodic wrap, and its offsets fit an int
+};
+
+// Each wave reads the parity and awake words itself (lane l: block l of the
+// 4 x 4 x 4 around the superblock's corner block), so no barrier is needed
+// before the box loads.
+template <int SZ>
+__device__ __forceinline__ MbSuper sb_setup(CInst& I, int sb, bool regular) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ntx = I.ntx, nty = I.nty, nsx = ntx >> 1, nsy = nty >> 1;
+    const int sx = sb % nsx, sy = (sb / nsx) % nsy, sz = sb / (nsx * nsy);
+    MbSuper s;
+    s.X0 = sx * 8;
+    s.Y0 = sy * 8;
+    s.Z0 = sz * 4 * SZ;
+    s.tw = (2 * sx + (w & 1)) + (2 * sy + ((w >> 1) & 1)) * ntx + (SZ * sz + (w >> 2)) * ntx * nty;
+    // a one-block-thick superblock's box reaches oz = 1 at most (lanes past it read that layer)
+    const int oz = min(lane >> 4, SZ + 1) - 1;
+    const int bx = wrap1(2 * sx + (lane & 3) - 1, ntx), by = wrap1(2 * sy + ((lane >> 2) & 3) - 1, nty),
+              bz = wrap1(SZ * sz + oz, I.ntz);
+    const int tt = bx + by * ntx + bz * ntx * nty;
+    const gu64 *par0 = I.par[0], *cur0 = I.a_cur[0];
+    asm volatile("" : "+s"(par0), "+s"(cur0));
+    const uint64_t wp = par0[tt >> 6], wa = cur0[tt >> 6];
+    constexpr uint64_t kOwn = Sb<SZ>::own();
+    s.pm = __ballot((int)((wp >> (tt & 63)) & 1ull));
+    s.am = regular ? kOwn : (__ballot((int)((wa >> (tt & 63)) & 1ull)) & kOwn);
+    s.interior = s.X0 > 0 && s.X0 + 8 < I.nx && s.Y0 > 0 && s.Y0 + 8 < I.ny && s.Z0 > 0 &&
+                 s.Z0 + 4 * SZ < I.nz && (int64_t)I.nx * I.ny < (1 << 27);
+    return s;
+}
+
+// The superblocks of this workgroup, k = 0, 1, ...: the band sweep of the block
+// kernels over superblock columns (XCD b mod 8 sweeps its band of columns
+// z-layer by z-layer), or plain grid-stride order where the counts do not allow it.
+template <int SZ>
+__device__ __forceinline__ int sb_of(CInst& I, int k) {
+    const int G = gridDim.x, nsb = I.nt / (4 * SZ), C = (I.ntx >> 1) * (I.nty >> 1);
+    if ((G & 7) == 0 && (C & 7) == 0) {
+        const int cx = C >> 3;
+        const int64_t i = (int64_t)(blockIdx.x >> 3) + (int64_t)k * (G >> 3);
+        if (i >= (nsb >> 3)) return -1;
+        return (int)(i / cx) * C + (int)(blockIdx.x & 7) * cx + (int)(i % cx);
+    }
+    const int64_t i = (int64_t)blockIdx.x + (int64_t)k * G;
+    return i < nsb ? (int)i : -1;
+}
+
+// The thread's box elements k = tid + e NT (the last round's slots past the box
+// hold element tid's coordinates and are neither loaded nor stored).
+template <int SZ>
+struct SbElems {
+    using S = Sb<SZ>;
+    int lx[S::NE], ly[S::NE], lz[S::NE], d[S::NE], off[S::NE];
+    __device__ __forceinline__ void init(int tid, int nx, int nxy) {
+#pragma unroll
+        for (int e = 0; e < S::NE; e++) {
+            const int k0 = tid + e * S::NT, k = k0 < S::A ? k0 : tid;
+            lx[e] = k % S::W;
+            ly[e] = (k / S::W) % S::W;
+            lz[e] = k / (S::W * S::W);
+            d[e] = ((lx[e] + 3) >> 2) + 4 * ((ly[e] + 3) >> 2) + 16 * ((lz[e] + 3) >> 2);
+            off[e] = lx[e] + ly[e] * nx + lz[e] * nxy;
+        }
+    }
+    // wrapped global index of element e of the box at (X0, Y0, Z0)
+    __device__ __forceinline__ int64_t index(CInst& I, const MbSuper& s, int e) const {
+        return wrap1(s.X0 + lx[e] - 1, I.nx) + (int64_t)wrap1(s.Y0 + ly[e] - 1, I.ny) * I.nx +
+               (int64_t)wrap1(s.Z0 + lz[e] - 1, I.nz) * ((int64_t)I.nx * I.ny);
+    }
+};
+
+// Calc_mu(_wakeup) + the detector on superblocks (see modelb_mu_kernel)
+// 4 waves per SIMD (128 VGPRs; left free the compiler takes 129 and 3 waves):
+// measured 0.945 ms per 256^3 step with the update kernel at 5, 1.04 ms at 3
+template <int SZ>
+__global__ __launch_bounds__(Sb<SZ>::NT) __attribute__((amdgpu_waves_per_eu(4))) void modelb_mu_sb_kernel(const WorkList wl, int sb_from) {
+    using S = Sb<SZ>;
+    constexpr int A = S::A, NT = S::NT, NE = S::NE;
+    __shared__ double sc[3 * A];
+    const int total = wl.offsets[wl.nseg];
+    if (total < sb_from) return;
+    CInst& I = *(CInst*)wl.insts;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const bool regular = I.mode == 0;
+    const int nx = I.nx, nxy = I.nx * I.ny;
+    SbElems<SZ> el;
+    el.init(tid, nx, nxy);
+    const int cx = 4 * (w & 1) + (lane & 3), cy = 4 * ((w >> 1) & 1) + ((lane >> 2) & 3), cz = 4 * (w >> 2) + (lane >> 4);
+    const int code = sb_code(w);
+    for (int k = 0;; k++) {
+        const int sb = sb_of<SZ>(I, k);
+        if (sb < 0) break;
+        const MbSuper s = sb_setup<SZ>(I, sb, regular);
+        if (!s.am) continue;
+        const bool awake = (s.am >> code) & 1ull;
+        const int pw = (int)(s.pm >> code) & 1;
+        const int64_t own = (s.X0 + cx) + (int64_t)(s.Y0 + cy) * nx + (int64_t)(s.Z0 + cz) * nxy;
+        double h[NE][3];
+        if (s.interior && (s.pm == 0ull || s.pm == ~0ull)) {
+            // no wrap, and every block the box touches current in one buffer
+            const int p = (int)(s.pm & 1ull);
+            const int64_t base = (s.X0 - 1) + (int64_t)(s.Y0 - 1) * nx + (int64_t)(s.Z0 - 1) * nxy;
+            const gdouble *b0 = I.buf[p][0] + base, *b1 = I.buf[p][1] + base, *b2 = I.buf[p][2] + base;
+#pragma unroll
+            for (int e = 0; e < NE; e++) {
+                if (e + 1 < NE || tid + e * NT < A) {
+                    h[e][0] = b0[el.off[e]];
+                    h[e][1] = b1[el.off[e]];
+                    h[e][2] = b2[el.off[e]];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < NE; e++) {
+                if (e + 1 < NE || tid + e * NT < A) {
+                    const int64_t gi = el.index(I, s, e);
+                    const int pp = (int)(s.pm >> el.d[e]) & 1;
+                    for (int f = 0; f < 3; f++) h[e][f] = I.buf[pp][f][gi];
+                }
+            }
+        }
+        double phi = 0.0;
+        if (awake) phi = I.buf[pw][3][own];
+#pragma unroll
+        for (int e = 0; e < NE; e++)
+            if (tid + e * NT < A)
+                for (int f = 0; f < 3; f++) sc[f * A + tid + e * NT] = h[e][f];
+        __syncthreads();
+        if (awake) {
+            double df[3];
+            const double lz_ = mb_mu_at<S::W, S::W * S::W, A>(I, sc, cx + 1, cy + 1, cz + 1, phi, df);
+            I.aux[0][own] = df[0];
+            I.aux[1][own] = df[1];
+            I.aux[2][own] = df[2];
+            if (I.mode != 0) {
+                const int any = __ballot(fabs(lz_) > I.thresh_a) != 0ull;
+                if (lane == 0) I.nflag[0][s.tw] = (uint8_t)any;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Update(_wakeup) + copy-back on superblocks (see modelb_update_kernel)
+// 5 waves per SIMD (96 VGPRs, no spills; 4 left free)
+template <int SZ>
+__global__ __launch_bounds__(Sb<SZ>::NT) __attribute__((amdgpu_waves_per_eu(5))) void modelb_update_sb_kernel(const WorkList wl, int sb_from) {
+    using S = Sb<SZ>;
+    constexpr int A = S::A, NT = S::NT, NE = S::NE;
+    __shared__ double smu[3 * A];
+    const int total = wl.offsets[wl.nseg];
+    if (total < sb_from) return;
+    CInst& I = *(CInst*)wl.insts;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const bool regular = I.mode == 0;
+    const int nx = I.nx, nxy = I.nx * I.ny;
+    SbElems<SZ> el;
+    el.init(tid, nx, nxy);
+    const int cx = 4 * (w & 1) + (lane & 3), cy = 4 * ((w >> 1) & 1) + ((lane >> 2) & 3), cz = 4 * (w >> 2) + (lane >> 4);
+    const int code = sb_code(w);
+    for (int k = 0;; k++) {
+        const int sb = sb_of<SZ>(I, k);
+        if (sb < 0) break;
+        const MbSuper s = sb_setup<SZ>(I, sb, regular);
+        if (!s.am) continue;
+        const bool awake = (s.am >> code) & 1ull;
+        const int pw = (int)(s.pm >> code) & 1;
+        const int64_t own = (s.X0 + cx) + (int64_t)(s.Y0 + cy) * nx + (int64_t)(s.Z0 + cz) * nxy;
+        double h[NE][3];
+        if (s.interior) {
+            const int64_t base = (s.X0 - 1) + (int64_t)(s.Y0 - 1) * nx + (int64_t)(s.Z0 - 1) * nxy;
+            const gdouble *a0 = I.aux[0] + base, *a1 = I.aux[1] + base, *a2 = I.aux[2] + base;
+#pragma unroll
+            for (int e = 0; e < NE; e++) {
+                if (e + 1 < NE || tid + e * NT < A) {
+                    h[e][0] = a0[el.off[e]];
+                    h[e][1] = a1[el.off[e]];
+                    h[e][2] = a2[el.off[e]];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < NE; e++) {
+                if (e + 1 < NE || tid + e * NT < A) {
+                    const int64_t gi = el.index(I, s, e);
+                    for (int f = 0; f < 3; f++) h[e][f] = I.aux[f][gi];
+                }
+            }
+        }
+        double v[3] = {0.0, 0.0, 0.0};
+        if (awake)
+            for (int f = 0; f < 3; f++) v[f] = I.buf[pw][f][own];
+#pragma unroll
+        for (int e = 0; e < NE; e++)
+            if (tid + e * NT < A)
+                for (int f = 0; f < 3; f++) smu[f * A + tid + e * NT] = h[e][f];
+        __syncthreads();
+        if (awake) {
+            double ch[3];
+            mb_change_at<S::W, S::W * S::W, A>(I, smu, cx + 1, cy + 1, cz + 1, ch);
+            I.buf[pw ^ 1][0][own] = v[0] + ch[0];
+            I.buf[pw ^ 1][1][own] = v[1] + ch[1];
+            I.buf[pw ^ 1][2][own] = v[2] + ch[2];
+        }
+        __syncthreads();
     }
 }
 

@@ -133,3 +133,45 @@ def test_modelb_reference_size_runs(ctx):
     b = 1.0 - inst.download(0) - inst.download(1) - inst.download(2)
     assert b.min() > 0.0
     inst.destroy()
+
+
+@pytest.mark.parametrize("mode", [abi.MODE_REGULAR, abi.MODE_DBA])
+def test_modelb_superblocks_match_block_kernels(ctx, mode, monkeypatch):
+    """The superblock kernels (2x2x2 blocks per workgroup) against the block
+    kernels on the same steps: DBA_MODELB_SUPERBLOCKS=1 takes every step on
+    superblocks, including the sparse ones with sleeping blocks inside a
+    superblock, =0 none; fields bit-identical, same awake sets and history,
+    and the superblock run within the bar of the oracle."""
+    n, nz = 48, 24
+    X = list(configs.modelb_ic(n, n, nz)) + [np.zeros((nz, n, n))]
+    if mode != abi.MODE_REGULAR:
+        X[0][..., n // 2:] = 0.2
+        X[1][..., n // 2:] = 0.2
+        X[2][..., n // 2:] = 0.01
+    p = configs.modelb_params(n=n, nz=nz, mode=mode, eps=1e-4, history_cap=64)
+    runs = {}
+    for env in ("1", "0"):
+        monkeypatch.setenv("DBA_MODELB_SUPERBLOCKS", env)
+        inst = ctx.create(p)
+        for f, a in enumerate(X):
+            inst.upload(f, a)
+        snaps = []
+        for chunk in (1, 4, 15):
+            inst.step(chunk)
+            snaps.append(([inst.download(f) for f in range(3)], inst.active(0)[1]))
+        runs[env] = (snaps, inst.history())
+        inst.destroy()
+    for (fa, aa), (fb, ab) in zip(runs["1"][0], runs["0"][0]):
+        for f in range(3):
+            assert np.array_equal(fa[f], fb[f]), f
+        assert np.array_equal(aa, ab)
+    assert np.array_equal(runs["1"][1], runs["0"][1])
+    o = OracleSim(p)
+    for f, a in enumerate(X):
+        o.set_field(f, a)
+    o.step(20)
+    for f in range(3):
+        assert np.max(np.abs(runs["1"][0][-1][0][f] - o.get_field(f))) <= TOL
+    if mode != abi.MODE_REGULAR:
+        h = runs["1"][1]
+        assert h[:, 0].min() < p.nx * p.ny * p.nz // 64   # sleeping blocks on some step
