@@ -26,7 +26,15 @@ the timed region. value = active cell-updates/s over the timed steps (device
 counter of cells in awake tiles).
 
 Before the timed region the graph it replays is captured, instantiated and
-replayed once, whatever --warmup is, so the timed region only replays it.
+replayed once, whatever --warmup is, so the timed region only replays it. Then
+a second instance of the same workload, from the same initial state, replays its
+own chunk graph, untimed, for at least --settle seconds (0.2 by default): the
+GPU's clocks take about that long to reach their steady state, and a few
+warm-up steps of a few microseconds each do not get there (CH 1024^2 with
+--steps 20 --warmup 5: 9.3 us per step without, 8.5 with). The measured
+instance has then still taken exactly --warmup steps plus the one chunk, the
+line reports the settling steps ("settle_steps"), and the timed region runs
+exactly --steps steps of the measured instance.
 
 The default workload's line also carries "companions": the same measurement of
 dendrite, ch_cons, kh, modelb and dendrite3d at their own step counts, in the
@@ -491,13 +499,15 @@ def companion(o):
     """The fields of a companion measurement kept in the default line."""
     r = o["roofline"]
     return {"workload": o["config"]["workload"], "value": o["value"], "unit": o["unit"], "steps": o["steps"],
-            "warmup": o["warmup"], "ms_per_step": o["ms_per_step"], "gpu_ms_per_step": o["gpu_ms_per_step"],
+            "warmup": o["warmup"], "settle_steps": o.get("settle_steps"), "ms_per_step": o["ms_per_step"],
+            "gpu_ms_per_step": o["gpu_ms_per_step"],
             "active_fraction": o["active_fraction"], "full_grid_updates_per_s": o["full_grid_updates_per_s"],
             "finite": o["finite"],
             "roofline": {k: r[k] for k in ("achieved", "frac", "traffic", "b_alg_per_cell", "avg_kernel_us", "kernel")}}
 
 
-def measure(ctx, workload, K, W, chunk_cap, timing_steps, single_launch, n=None, tile=None, threads=0):
+def measure(ctx, workload, K, W, chunk_cap, timing_steps, single_launch, n=None, tile=None, threads=0,
+            settle=0.2):
     """One instance of `workload`: W warm-up steps, then K timed steps replayed
     from a captured graph; returns the JSON line without cpu_baseline."""
     w = WORKLOADS[workload]
@@ -514,6 +524,19 @@ def measure(ctx, workload, K, W, chunk_cap, timing_steps, single_launch, n=None,
     # whatever W is, so the timed region launches a cached, warm graph only.
     inst.step(chunk, graph_chunk=chunk)
     ctx.sync()
+    # clock settling on a second instance of the same workload, from the same
+    # initial state, so the measured instance keeps the state of its own warm-up
+    settle_steps = 0
+    warm = None
+    if settle > 0:
+        warm = ctx.create(p)
+        for f, a in enumerate(fields):
+            warm.upload(f, a)
+        ts = time.perf_counter()
+        while time.perf_counter() - ts < settle:
+            warm.step(chunk, graph_chunk=chunk)
+            ctx.sync()
+            settle_steps += chunk
     info0 = inst.info()
     u0 = info0.cell_updates
     single = bool(info0.single_launch)
@@ -524,6 +547,8 @@ def measure(ctx, workload, K, W, chunk_cap, timing_steps, single_launch, n=None,
     ctx.sync()
     t1 = time.perf_counter()
     u1 = inst.info().cell_updates
+    if warm is not None:
+        warm.destroy()
     gpu_ms = ctx.elapsed_ms(0, 1)
     wall_s = t1 - t0
     active_updates = u1 - u0
@@ -576,6 +601,7 @@ def measure(ctx, workload, K, W, chunk_cap, timing_steps, single_launch, n=None,
         "n_gpus": 1,
         "steps": K,
         "warmup": W,
+        "settle_steps": settle_steps,
         "ms_per_step": wall_s * 1e3 / K,
         "higher_is_better": True,
         "scaling": "weak",
@@ -621,6 +647,8 @@ def main():
     ap.add_argument("--no-companions", action="store_true",
                     help="default workload: skip the other single-instance configurations")
     ap.add_argument("--timing-steps", type=int, default=500)
+    ap.add_argument("--settle", type=float, default=0.2,
+                    help="untimed replays of the timed graph before the timed region, in seconds")
     ap.add_argument("--n", type=int, default=None, help="grid side (ch, ch_cons, dendrite, kh); default: the workload's")
     ap.add_argument("--tile", type=int, default=None, help="activation tile edge override")
     ap.add_argument("--threads", type=int, default=0, help="workgroup size override (64/128/256)")
@@ -655,10 +683,10 @@ def main():
         return
     with Context(0) as ctx:
         out = measure(ctx, args.workload, K, W, args.chunk, args.timing_steps, args.single_launch,
-                      n=args.n, tile=args.tile, threads=args.threads)
+                      n=args.n, tile=args.tile, threads=args.threads, settle=args.settle)
         if args.workload == "ch" and not (args.n or args.tile or args.threads or args.no_companions):
             out["companions"] = [companion(measure(ctx, c, WORKLOADS[c]["steps"], WORKLOADS[c]["warm"], args.chunk,
-                                                   args.timing_steps, args.single_launch))
+                                                   args.timing_steps, args.single_launch, settle=args.settle))
                                  for c in COMPANIONS]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.workload, n=args.n)
