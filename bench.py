@@ -20,6 +20,16 @@ Workloads (--workload):
             arrival rate steered toward 80% budget occupancy; reports p50/p99
             completion latency, the aggregate rate, and the plain sequential
             rate of a sample of the same requests run alone.
+  modelb    the reference's ternary Model B, 256^3 periodic FP64, 4x4x4 blocks,
+            DBA (every block stays awake); CPU baseline: the reference's own CPU
+            program compiled from its sources.
+  dendrite3d the reference's 3D Kobayashi-94 dendrite, 256^3 FP64, 4x4x4 blocks,
+            DBA (about 2% of the field-set updates); CPU baseline as modelb.
+  copycal   no DBA step: device-to-device copies of known size, for the traffic
+            counters' calibration.
+  hbmref    no DBA step: what a plain device copy (read + write) and sum (read)
+            of 256 MB - 2 GB FP64 arrays move on this GPU, the yardstick for
+            the step kernels' achieved bandwidth.
 A step is one DBA time step of the whole instance (stencil kernel + finalize
 kernel), replayed from a captured hipGraph. Inputs are resident in HBM before
 the timed region. value = active cell-updates/s over the timed steps (device
@@ -73,6 +83,7 @@ WORKLOADS = {
     "batch": dict(model="k93", b_alg=32, warm=0, steps=5000),
     "serve": dict(model="mix", b_alg=None, warm=0, steps=0),
     "copycal": dict(model="ch", b_alg=None, warm=0, steps=0),
+    "hbmref": dict(model=None, b_alg=None, warm=0, steps=0),
 }
 
 
@@ -321,6 +332,34 @@ def copy_calibration(n=4096, reps=8):
         inst.destroy()
     return {"workload": "copycal", "copies": reps, "bytes_per_copy": n * n * 8,
             "kernel": "__amd_rocclr_copyBuffer (D2D half of each upload)"}
+
+
+def hbm_reference(sizes_mb=(256, 1024, 2048), reps=50):
+    """What plain streaming moves on this GPU, as a yardstick for the step
+    kernels' achieved bandwidth: PyTorch's device copy (each byte read once and
+    written once, like a stencil step's 16 B per cell) and its sum (read only),
+    of FP64 arrays of the given sizes, timed with hipEvents over `reps` launches
+    after 5 untimed ones."""
+    import torch
+    dev = torch.device("cuda:0")
+    out = {"workload": "hbmref", "unit": "TB/s", "copy": {}, "sum": {}}
+    for mb in sizes_mb:
+        n = mb * 1024 * 1024 // 8
+        x = torch.rand(n, dtype=torch.float64, device=dev)
+        y = torch.empty_like(x)
+        for name, fn, nbytes in (("copy", lambda: y.copy_(x), 2 * n * 8), ("sum", lambda: x.sum(), n * 8)):
+            for _ in range(5):
+                fn()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            out[name][f"{mb}MB"] = nbytes / (e0.elapsed_time(e1) / reps / 1e3) / 1e12
+        del x, y
+    return out
 
 
 def traffic_from_profile(workload):
@@ -677,6 +716,9 @@ def main():
         return
     if args.workload == "serve":
         print(json.dumps(bench_serve(args)))
+        return
+    if args.workload == "hbmref":
+        print(json.dumps(hbm_reference()))
         return
     if args.workload == "copycal":
         print(json.dumps(copy_calibration()))
