@@ -335,6 +335,30 @@ __device__ __forceinline__ int find_slot(const int32_t* offsets, int n, int item
     return lo;
 }
 
+// The same segment for an item every lane of the wave holds: the offsets are
+// non-decreasing, so the last segment k with offsets[k] <= item is the number of
+// k in [1, n) with offsets[k] <= item. The wave reads 256 offsets per round trip
+// (four per lane) and counts with ballots, where the binary search made
+// log2(n) dependent loads (8 for a 256-segment list).
+__device__ __forceinline__ int find_slot_wave(const int32_t* offsets, int n, int item) {
+    const int lane = threadIdx.x & 63;
+    int cnt = 0;
+    for (int k0 = 1; k0 < n; k0 += 256) {
+        int o[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int k = k0 + j * 64 + lane;
+            o[j] = offsets[k < n ? k : 0];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int k = k0 + j * 64 + lane;
+            cnt += __popcll(__ballot(k < n && o[j] <= item));
+        }
+    }
+    return cnt;
+}
+
 // Current value of a field at cell i: the buffer the tile parity selects,
 // minus the conserved shift accumulated since the tile last ran.
 __device__ __forceinline__ double current_value(CInst& I, int field, int64_t i) {
@@ -381,7 +405,7 @@ __device__ __forceinline__ bool next_item(const WorkList& w, int item, int total
         return true;
     }
     if (item >= total) return false;
-    const int s = w.nseg == 1 ? 0 : find_slot(w.offsets, w.nseg, item);
+    const int s = w.nseg == 1 ? 0 : find_slot_wave(w.offsets, w.nseg, item);
     const Seg sg = w.segs[s];
     *I = (CInst*)w.insts + __builtin_amdgcn_readfirstlane(sg.inst);
     const int pos = __builtin_amdgcn_readfirstlane(sg.tile0 + (item - w.offsets[s]));
