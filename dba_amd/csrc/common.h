@@ -340,11 +340,12 @@ __device__ __forceinline__ int find_slot(const int32_t* offsets, int n, int item
 // k in [1, n) with offsets[k] <= item. The wave reads 256 offsets per round trip
 // (four per lane) and counts with ballots, where the binary search made
 // log2(n) dependent loads (8 for a 256-segment list).
-__device__ __forceinline__ int find_slot_wave(const int32_t* offsets, int n, int item) {
+// With at most 257 segments (one round) the segment's offset is taken from the
+// lane that loaded it, so the item's position needs no further load of it.
+__device__ __forceinline__ int find_slot_wave(const int32_t* offsets, int n, int item, int* off_s) {
     const int lane = threadIdx.x & 63;
-    int cnt = 0;
+    int cnt = 0, o[4] = {0, 0, 0, 0};
     for (int k0 = 1; k0 < n; k0 += 256) {
-        int o[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int k = k0 + j * 64 + lane;
@@ -355,6 +356,15 @@ __device__ __forceinline__ int find_slot_wave(const int32_t* offsets, int n, int
             const int k = k0 + j * 64 + lane;
             cnt += __popcll(__ballot(k < n && o[j] <= item));
         }
+    }
+    if (cnt == 0) {
+        *off_s = 0;  // offsets[0]
+    } else if (n <= 257) {
+        const int i = cnt - 1, j = i >> 6;
+        const int v = j == 0 ? o[0] : (j == 1 ? o[1] : (j == 2 ? o[2] : o[3]));
+        *off_s = __builtin_amdgcn_readlane(v, i & 63);
+    } else {
+        *off_s = offsets[cnt];
     }
     return cnt;
 }
@@ -385,6 +395,7 @@ struct WorkList {
     const int32_t* solo_seg;  // single-instance single-segment groups: its list, else null
     const uint64_t* solo_rec; // ... its neighbour records
     int solo_nt;              // ... and its tile count (0 otherwise)
+    int one_inst;             // one instance: segment s holds its tiles from s kSegTiles on
 };
 
 // Item -> (instance, tile, neighbour record); false once the item is past the
@@ -405,10 +416,16 @@ __device__ __forceinline__ bool next_item(const WorkList& w, int item, int total
         return true;
     }
     if (item >= total) return false;
-    const int s = w.nseg == 1 ? 0 : find_slot_wave(w.offsets, w.nseg, item);
-    const Seg sg = w.segs[s];
-    *I = (CInst*)w.insts + __builtin_amdgcn_readfirstlane(sg.inst);
-    const int pos = __builtin_amdgcn_readfirstlane(sg.tile0 + (item - w.offsets[s]));
+    int s = 0, off = 0;
+    if (w.nseg > 1) s = find_slot_wave(w.offsets, w.nseg, item, &off);
+    int inst = 0, tile0 = s * kSegTiles;
+    if (!w.one_inst) {
+        const Seg sg = w.segs[s];
+        inst = sg.inst;
+        tile0 = sg.tile0;
+    }
+    *I = (CInst*)w.insts + __builtin_amdgcn_readfirstlane(inst);
+    const int pos = __builtin_amdgcn_readfirstlane(tile0 + (item - off));
     *t = __builtin_amdgcn_readfirstlane((*I)->seg[pos]);
     const uint64_t r = want_rec ? (*I)->nbrec[pos] : 0ull;
     *rec = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)r)) |

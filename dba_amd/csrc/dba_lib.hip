@@ -558,7 +558,8 @@ dba_status get_group(dba_ctx* ctx, const std::vector<int>& ids, Group** out) {
 // ---------------------------------------------------------------- launches
 
 WorkList wl_of(const Group* g) {
-    return WorkList{g->d_insts, g->d_segs, g->d_offsets, g->nseg, g->solo_seg, g->solo_rec, g->solo_nt};
+    return WorkList{g->d_insts, g->d_segs, g->d_offsets, g->nseg, g->solo_seg, g->solo_rec, g->solo_nt,
+                    g->ids.size() == 1 ? 1 : 0};
 }
 
 template <int TB, int NT>
