@@ -739,7 +739,7 @@ void launch_dense(const Group* g, hipStream_t s, int first, int step) {
 
 void launch_finalize(const Group* g, hipStream_t s, int rotate) {
     hipLaunchKernelGGL(finalize_kernel, dim3((unsigned)g->nseg), dim3(kFinNT), 0, s, g->d_insts, g->d_segs,
-                       g->nseg, g->d_parts, g->d_counts, g->d_offsets, rotate);
+                       g->nseg, g->d_parts, g->d_counts, g->d_offsets, rotate, g->ids.size() == 1 ? 1 : 0);
     if (g->nseg > 1) {
         // workgroup 0 scans the segment counts; the others reduce 16 instances each
         const int n_inst = (int)g->ids.size();

@@ -151,7 +151,7 @@ template <int NT>
 __device__ __forceinline__ void finalize_body(const Inst* __restrict__ insts, const Seg* __restrict__ segs,
                                               int segi, int nseg, SegPart* __restrict__ parts,
                                               int32_t* __restrict__ counts, int32_t* __restrict__ offsets,
-                                              int rotate, FinLds& F) {
+                                              int rotate, int one_inst, FinLds& F) {
     static_assert(NT == kSegTiles, "one thread per tile of a segment");
     auto& s_pb = F.s_pb;
     auto& s_ab = F.s_ab;
@@ -165,10 +165,15 @@ __device__ __forceinline__ void finalize_body(const Inst* __restrict__ insts, co
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     // one segment: instance 0, all of its tiles (no dependent load of the segment table)
-    const int inst = nseg == 1 ? 0 : __builtin_amdgcn_readfirstlane(segs[segi].inst);
+    // one instance: segment segi holds its tiles from segi kSegTiles on, so the
+    // segment table is not read (its load would come before the descriptor's)
+    const bool one = nseg == 1 || one_inst;
+    const int inst = one ? 0 : __builtin_amdgcn_readfirstlane(segs[segi].inst);
     CInst& I = ((CInst*)insts)[inst];
-    const int t0 = nseg == 1 ? 0 : __builtin_amdgcn_readfirstlane(segs[segi].tile0);
-    const int nts = nseg == 1 ? I.nt : __builtin_amdgcn_readfirstlane(segs[segi].ntile);
+    const int t0 = nseg == 1 ? 0 : (one_inst ? segi * kSegTiles : __builtin_amdgcn_readfirstlane(segs[segi].tile0));
+    const int nts = nseg == 1 ? I.nt
+                              : (one_inst ? min(kSegTiles, I.nt - t0)
+                                          : __builtin_amdgcn_readfirstlane(segs[segi].ntile));
     const int w0 = t0 >> 6, nws = (nts + 63) >> 6;
     const bool regular = I.mode == 0;
     const int nsets = I.nsets;
@@ -409,9 +414,10 @@ __global__ __launch_bounds__(kFinNT) void finalize_kernel(const Inst* __restrict
                                                           const Seg* __restrict__ segs, int nseg,
                                                           SegPart* __restrict__ parts,
                                                           int32_t* __restrict__ counts,
-                                                          int32_t* __restrict__ offsets, int rotate) {
+                                                          int32_t* __restrict__ offsets, int rotate,
+                                                          int one_inst) {
     __shared__ FinLds F;
-    finalize_body<kFinNT>(insts, segs, blockIdx.x, nseg, parts, counts, offsets, rotate, F);
+    finalize_body<kFinNT>(insts, segs, blockIdx.x, nseg, parts, counts, offsets, rotate, one_inst, F);
 }
 
 __global__ __launch_bounds__(kFinNT) void finalize_merge_kernel(const Inst* __restrict__ insts, int nseg,
