@@ -444,20 +444,24 @@ __global__ __launch_bounds__(NT) void ch_dense_kernel(const Inst* __restrict__ i
     const bool regular = I.mode == 0;
     constexpr bool cons = CONS;  // conserved DBA: one instance in one segment (the host checks)
     const int runs = load_early(I.druns + t);  // read early, off the tail (every lane: no branch)
-    dense_nb_fill(nb, I, t, first, prev, regular, 1, cons);
+    // The previous step's change sums (tpart + (1 + parity) nt, 0 for tiles that
+    // did not run) in the finalize's order: 16 words of 64 tiles, each by the
+    // shuffle tree of block_sum, then the words in order. Their loads go out
+    // before the neighbour table's, so the two round trips overlap.
+    constexpr int NWV = NT / 64, PER = (kFinNT / 64 + NWV - 1) / NWV;
+    double v[PER];
     if (cons && !first) {
-        // The previous step's change sums (tpart + (1 + parity) nt, 0 for tiles
-        // that did not run) in the finalize's order: 16 words of 64 tiles, each
-        // by the shuffle tree of block_sum, then the words in order.
-        constexpr int NWV = NT / 64, PER = (kFinNT / 64 + NWV - 1) / NWV;
         const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
         const gdouble* tp = I.tpart + (int64_t)(1 + prev) * I.nt;
-        double v[PER];
 #pragma unroll
         for (int k = 0; k < PER; k++) {  // all of this wave's words in flight at once
             const int w = wid + k * NWV, tt = w * 64 + lane;
             v[k] = (w < kFinNT / 64 && tt < I.nt) ? tp[tt] : 0.0;
         }
+    }
+    dense_nb_fill(nb, I, t, first, prev, regular, 1, cons);
+    if (cons && !first) {
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
         for (int k = 0; k < PER; k++) {
             v[k] = wave_sum_down(v[k]);  // block_sum's tree
