@@ -46,6 +46,26 @@ def test_batched_equals_alone(ctx, n):
         inst.destroy()
 
 
+def test_batched_many_segments_equals_alone(ctx):
+    """300 small instances, one segment each: a merged list of more than 257
+    segments, whose lookup counts the segment offsets in two rounds of loads and
+    reads the item's segment offset back from memory, and a merge of 19
+    workgroups of instance reductions."""
+    items = batch.make_sweep(n=300, master_seed=7, side_lo=48, side_hi=96, multiple=16)
+    st, insts = batch.run_batched(ctx, items, 60, tile=16, chunk=30, keep=True)
+    assert st.active_updates > 0
+    for it, inst in zip(items, insts):
+        (alone,) = batch._create(ctx, [it], 16, 0)
+        alone.step(60, graph_chunk=30)
+        for a, b in zip(fields_of(inst), fields_of(alone)):
+            assert np.array_equal(a, b), it.index
+        assert np.array_equal(inst.active(0)[1], alone.active(0)[1])
+        assert np.array_equal(inst.active(1)[1], alone.active(1)[1])
+        assert inst.info().cell_updates == alone.info().cell_updates
+        alone.destroy()
+        inst.destroy()
+
+
 def test_batched_matches_oracle(ctx):
     items = [it for it in small_sweep(10) if it.aniso_j == 6][:1] + \
             [it for it in small_sweep(10) if it.aniso_j == 4][:1]
