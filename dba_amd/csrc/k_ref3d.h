@@ -822,32 +822,47 @@ __global__ __launch_bounds__(64) void k94_step_kernel(const WorkList wl) {
         __syncthreads();
         const int ap = nb.act[0][13], at = nb.act[1][13];
         const int pp0 = nb.par[0][13], pp1 = nb.par[1][13];
-        // stage phi and T of the block and its 1-cell halo: every load issued
-        // before the first wait (a cell not needed loads from its valid address
-        // anyway and is not stored), from buffer pointers of one scalar batch
+        // stage phi and T of the block and the face cells of its 1-cell halo (the
+        // 7-point stencils read no edge or corner of the box: 160 of its 216 cells
+        // at 4x4x4): every load issued before the first wait (a slot past the cells
+        // loads cell 0 of the list and is not stored), from buffer pointers of one
+        // scalar batch. The cells: the block's column with its two z faces, then the
+        // two x faces, then the two y faces.
         {
-            constexpr int KA = (A + NT - 1) / NT;
+            constexpr int NCOL = TB * TB * (TZ + 2), NF = TB * TZ, NC = NCOL + 4 * NF;
+            constexpr int KA = (NC + NT - 1) / NT;
             const gdouble *bp0 = I.buf[0][0], *bp1 = I.buf[1][0], *bT0 = I.buf[0][1], *bT1 = I.buf[1][1];
             asm volatile("" : "+s"(bp0), "+s"(bp1), "+s"(bT0), "+s"(bT1));
             double vp[KA], vT[KA];
-            int dk[KA];
+            int dk[KA], kb[KA];
 #pragma unroll
             for (int q = 0; q < KA; q++) {
-                const int k0 = tid + q * NT, k = k0 < A ? k0 : 0;
-                const int lx = k % WX, ly = (k / WX) % WX, lz = k / (WX * WX);
+                const int k0 = tid + q * NT, k = k0 < NC ? k0 : 0;
+                int lx, ly, lz;
+                if (k < NCOL) {
+                    lx = 1 + k % TB;
+                    ly = 1 + (k / TB) % TB;
+                    lz = k / (TB * TB);
+                } else {
+                    const int j = k - NCOL, face = j / NF, jj = j % NF;
+                    const int u = 1 + jj % TB, edge = (face & 1) ? TB + 1 : 0;
+                    lz = 1 + jj / TB;
+                    lx = face < 2 ? edge : u;
+                    ly = face < 2 ? u : edge;
+                }
                 const int d = nb_dir3<TB, TZ, 1>(lx, ly, lz);
                 const int64_t gi = wrapi(x0 + lx - 1, nx) + wrapi(y0 + ly - 1, ny) * (int64_t)nx +
                                    wrapi(z0 + lz - 1, nz) * nxy;
                 vp[q] = (nb.par[0][d] ? bp1 : bp0)[gi];
                 vT[q] = (nb.par[1][d] ? bT1 : bT0)[gi];
                 dk[q] = d;
+                kb[q] = lx + ly * WX + lz * WX * WX;
             }
 #pragma unroll
             for (int q = 0; q < KA; q++) {
-                const int k = tid + q * NT;
-                if (k < A) {
-                    if (dk[q] == 13 || ap) sp[k] = vp[q];
-                    if (dk[q] == 13 || at || regular) sT[k] = vT[q];
+                if (tid + q * NT < NC) {
+                    if (dk[q] == 13 || ap) sp[kb[q]] = vp[q];
+                    if (dk[q] == 13 || at || regular) sT[kb[q]] = vT[q];
                 }
             }
         }
