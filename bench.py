@@ -703,8 +703,9 @@ def main():
     ap.add_argument("--seq-budget-s", type=float, default=30.0, help="serve: time for the sequential sample")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported in this build "
-              "(instances shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
+              "(one context per GPU; independent instances would shard across GPUs with no exchange)",
+              file=sys.stderr)
         sys.exit(2)
 
     from dba_amd import Context
