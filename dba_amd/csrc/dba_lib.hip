@@ -107,7 +107,6 @@ struct Group {
     bool spec = false;            // K93 single-launch steps with speculative staging
     bool kh_p2 = false;           // KH: dx, dy and the cell area of every instance are powers of two
     int sb_from = 0;              // Model B: lists of at least this many blocks step on superblocks (0: never)
-    int sb_z = 1;                 // ... of 2 x 2 x sb_z blocks
     bool nine = true;             // CH 9-point stencil (the march kernel is instanced on it)
     Inst* d_insts = nullptr;
     Seg* d_segs = nullptr;
@@ -506,13 +505,13 @@ dba_status get_group(dba_ctx* ctx, const std::vector<int>& ids, Group** out) {
     int64_t cap = g->model == DBA_MODEL_MODELB ? 65536 : (g->model == DBA_MODEL_DENDRITE_K94 ? 16384 : 4096);
     g->grid = (int)std::max<int64_t>(1, std::min<int64_t>(g->total_tiles, cap));
     // Model B superblocks (modelb_mu_sb_kernel): one 3D instance of full 4x4x4
-    // blocks, an even number of blocks per axis, no conserved shift; steps with
+    // blocks, an even number of blocks in x and y, no conserved shift; steps with
     // at least a quarter of the blocks awake. DBA_MODELB_SUPERBLOCKS=0 turns them
     // off and =1 uses them for every non-empty list (both give the same doubles).
     if (g->model == DBA_MODEL_MODELB && ids.size() == 1 && g->tile == 4 && g->tile_z == 4) {
         const Inst& h = first->h;
         if (first->p.mode != DBA_MODE_CONSERVED && h.full_tiles && h.nz > 1 && h.ntx % 2 == 0 &&
-            h.nty % 2 == 0 && h.ntz % g->sb_z == 0) {
+            h.nty % 2 == 0) {
             const char* env = std::getenv("DBA_MODELB_SUPERBLOCKS");
             const std::string e = env ? env : "";
             g->sb_from = e == "0" ? 0 : (e == "1" ? 1 : (int)((h.nt + 3) / 4));
@@ -609,23 +608,15 @@ void launch_step(const Group* g, hipStream_t s) {
     if (g->model == DBA_MODEL_DENDRITE_K94 || g->model == DBA_MODEL_MODELB) {
         dim3 grid(g->grid), block(64);
         if (g->model == DBA_MODEL_MODELB) {
-            // each pair returns at once when the other takes the step (by list length)
+            // each pair returns at once when the other takes the step (by list length);
+            // superblocks of 2 x 2 x 1 blocks (2 x 2 x 2 on 8-wave workgroups measured slower)
             const int from = g->sb_from > 0 ? g->sb_from : INT_MAX;
-            const dim3 sgrid((unsigned)std::max<int64_t>(1, g->total_tiles / (4 * g->sb_z))), sblock(256 * g->sb_z);
+            const dim3 sgrid((unsigned)std::max<int64_t>(1, g->total_tiles / 4)), sblock(Sb<1>::NT);
             if (g->tile_z == 4) {
-                if (g->sb_from > 0) {
-                    if (g->sb_z == 2)
-                        hipLaunchKernelGGL(modelb_mu_sb_kernel<2>, sgrid, sblock, 0, s, wl_of(g), from);
-                    else
-                        hipLaunchKernelGGL(modelb_mu_sb_kernel<1>, sgrid, sblock, 0, s, wl_of(g), from);
-                }
+                if (g->sb_from > 0) hipLaunchKernelGGL(modelb_mu_sb_kernel<1>, sgrid, sblock, 0, s, wl_of(g), from);
                 hipLaunchKernelGGL((modelb_mu_kernel<4, 4>), grid, block, 0, s, wl_of(g), from);
-                if (g->sb_from > 0) {
-                    if (g->sb_z == 2)
-                        hipLaunchKernelGGL(modelb_update_sb_kernel<2>, sgrid, sblock, 0, s, wl_of(g), from);
-                    else
-                        hipLaunchKernelGGL(modelb_update_sb_kernel<1>, sgrid, sblock, 0, s, wl_of(g), from);
-                }
+                if (g->sb_from > 0)
+                    hipLaunchKernelGGL(modelb_update_sb_kernel<1>, sgrid, sblock, 0, s, wl_of(g), from);
                 hipLaunchKernelGGL((modelb_update_kernel<4, 4>), grid, block, 0, s, wl_of(g), from);
             } else {
                 hipLaunchKernelGGL((modelb_mu_kernel<4, 1>), grid, block, 0, s, wl_of(g), from);
